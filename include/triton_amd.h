@@ -1,0 +1,185 @@
+/*
+ * triton_amd.h -- C ABI of the MI355X-native TRITON timestep.
+ *
+ * Drop-in boundary: the reference calls its timestep kernels as C++ templates
+ * Kernels::<name><T>(...) on Kokkos' default instance (src/kokkos_utils.h:18-40)
+ * from Triton::triton<T>::compute_local_dt / compute_new_state
+ * (src/triton_simulate.h:191-346).  Each tr_<name>_f64 below replaces one of
+ * them with the same argument order and meaning, device pointers owned by the
+ * caller, an explicit hipStream_t (passed as void*, NULL = null stream) and an
+ * int status (TR_OK or an error code; tr_last_error() gives the message of
+ * the calling thread's last failure).  Arrays are row-major rows x cols with
+ * the one-cell ghost ring, row 0 = north, exactly as the reference lays out
+ * device_vec (src/constants.h:82-112).
+ *
+ * Above the per-kernel functions sit the objects the reference has no single
+ * call for: a loaded case (the .cfg/DEM/roughness/hydrograph input formats),
+ * a device-resident simulation (the simulate() loop, src/triton_simulate.h:
+ * 12-159), a batch of ensemble members and a serving queue.
+ *
+ * Thread-compatibility: distinct objects may be driven from distinct host
+ * threads; one object must not be used by two threads at once.
+ */
+#ifndef TRITON_AMD_H
+#define TRITON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+#define TR_OK 0
+#define TR_ERR_INVALID 1   /* bad argument or unsupported option */
+#define TR_ERR_INPUT 2     /* unreadable or malformed input file */
+#define TR_ERR_HIP 3       /* HIP runtime error */
+#define TR_ERR_NOMEM 4     /* allocation failed or over the memory budget */
+#define TR_ERR_STATE 5     /* object in the wrong state for the call */
+
+/* Message of the calling thread's last error ("" if none). */
+const char *tr_last_error(void);
+/* Library version string. */
+const char *tr_version(void);
+
+/* ---- precision / execution form of a simulation -------------------------- */
+#define TR_FORM_F64_UNFUSED 0 /* reference kernel sequence, dt read back each step */
+#define TR_FORM_F64_FUSED 1   /* one fused launch per step, dt and time on device */
+#define TR_FORM_F32_FUSED 2   /* FP32 depth/discharge, FP64 time and volumes */
+
+/* ---- loaded case (input formats) ----------------------------------------- */
+typedef struct tr_case tr_case;
+
+/* Plain view of a case; every pointer is host memory owned by the case (or,
+ * when filled by the caller for tr_sim_create_view, by the caller).  Arrays
+ * are padded rows x cols unless stated. */
+typedef struct tr_case_view {
+  int32_t rows, cols;            /* padded dims: nrows_dem + 2, ncols_dem + 2 */
+  double dx;                     /* cell size (m) */
+  double xll, yll;               /* lower-left corner of the DEM */
+  double courant, hextra;
+  double *h, *qx, *qy;           /* initial state */
+  double *dem;                   /* bed with ghosts (walls 1e6 or copies) */
+  double *nman2;                 /* squared Manning n */
+  double *maxh;                  /* initial max depth */
+  int32_t *runoff_id;            /* zone per cell, -1 without runoff */
+  int32_t num_runoffs, roff_nrows, roff_row0;
+  double *roff_time;             /* [roff_nrows] s */
+  double *roff_rate;             /* [num_runoffs*roff_nrows] m/s, zone-major */
+  int32_t num_src, hyg_nrows;
+  double *hyg_time;              /* [hyg_nrows] s */
+  double *hyg_val;               /* [hyg_nrows*num_src] m^3/s, time-major */
+  int32_t *src_pos;              /* [num_src] flat padded cell index */
+  int32_t num_extbc_cells;
+  int32_t *bc_index, *bc_type, *bc_start, *bc_nrows;
+  int32_t bc_nvars;
+  double *bc_var1, *bc_var2;     /* [bc_nvars] */
+  int32_t open_boundaries;
+  double sim_start, sim_duration, print_interval, print_observation, init_dt;
+  int32_t time_series_flag, checkpoint_id, it_print;
+} tr_case_view;
+
+/* Parse a .cfg and load every input it names (src/triton_init.h:33-81).
+ * Relative paths in the .cfg are resolved against base_dir (NULL = cwd). */
+int tr_case_load(const char *cfg_path, const char *base_dir, tr_case **out);
+/* Fill `out` with pointers into the case (valid until tr_case_free). */
+int tr_case_view_get(tr_case *c, tr_case_view *out);
+void tr_case_free(tr_case *c);
+
+/* Reference output writers (src/output_ascii.h:52-75, src/output_binary.h:65-80):
+ * interior of a padded rows x cols host array. */
+int tr_write_output_ascii(const char *path, int32_t rows, int32_t cols, const double *a);
+int tr_write_output_binary(const char *path, int32_t rows, int32_t cols, const double *a);
+
+/* ---- per-kernel drop-in entry points (FP64, device pointers) ------------- */
+/* src/kernels.h:166-187 Kernels::compute_dt_and_sqrt */
+int tr_compute_dt_and_sqrt_f64(int32_t size, double dx, const double *qx, const double *qy,
+                               const double *h, double *sqrth, double *dt_out, double cn,
+                               double hextra, void *stream);
+/* src/kernels.h:195-212 Kernels::find_min_dt.  The min goes to the device
+ * scalar d_min; when h_min is non-NULL the call also copies it there and
+ * synchronises `stream` (the reference returns it by value). */
+int tr_find_min_dt_f64(int32_t size, const double *dtarr, double *d_min, double *h_min,
+                       void *stream);
+/* src/kernels_flux.h:31-285 Kernels::flux_x */
+int tr_flux_x_f64(int32_t size, int32_t nrows, int32_t ncols, double dx, double dt,
+                  const double *h, const double *qx, const double *qy, const double *dem,
+                  const double *sqrth, double *rhsh0, double *rhsh1, double *rhsqx0,
+                  double *rhsqx1, double *rhsqy0, double *rhsqy1, double hextra,
+                  void *stream);
+/* src/kernels_flux.h:308-559 Kernels::flux_y */
+int tr_flux_y_f64(int32_t size, int32_t nrows, int32_t ncols, double dx, double dt,
+                  const double *h, const double *qx, const double *qy, const double *dem,
+                  const double *sqrth, double *rhsh0, double *rhsh1, double *rhsqx0,
+                  double *rhsqx1, double *rhsqy0, double *rhsqy1, double hextra,
+                  void *stream);
+/* src/kernels_state.h:30-104 Kernels::update_cells */
+int tr_update_cells_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, double *h,
+                        double *qx, double *qy, const double *dem, const double *nman,
+                        double *rhsh0, double *rhsh1, double *rhsqx0, double *rhsqx1,
+                        double *rhsqy0, double *rhsqy1, double hextra, void *stream);
+/* src/kernels.h:102-152 Kernels::update_runoff */
+int tr_update_runoff_f64(int32_t size, int32_t nrows, int32_t ncols, double dt,
+                         const int32_t *runoff_id, int32_t index_row_runoff,
+                         int32_t n_rows_runoff, const double *runoff_intensity, double *h,
+                         double *qx, double *qy, double hextra, void *stream);
+/* src/kernels.h:50-84 Kernels::compute_flow (sources applied in list order) */
+int tr_compute_flow_f64(int32_t size, const double *hyg_time, const double *hyg_val,
+                        double dx, double dt, double simtime, int32_t idx_low,
+                        int32_t idx_high, double *h, const int32_t *pos, void *stream);
+/* src/kernels.h:436-460 Kernels::copy_info_to_exterior_boundaries_west_east */
+int tr_open_bc_west_east_f64(int32_t size, int32_t nrows, int32_t ncols, double *h,
+                             double *qx, double *qy, void *stream);
+/* src/kernels.h:476-512 Kernels::copy_info_to_exterior_boundaries_north_south */
+int tr_open_bc_north_south_f64(int32_t size, int32_t nrows, int32_t ncols, double *h,
+                               double *qx, double *qy, int32_t rank, int32_t total_process,
+                               void *stream);
+/* src/kernels.h:236-424 Kernels::compute_extbc_values (cells in list order) */
+int tr_compute_extbc_values_f64(int32_t size, int32_t nrows, int32_t ncols, double dt,
+                                double *h, double *qx, double *qy, const double *dem,
+                                const double *nman, const int32_t *relative_index,
+                                const int32_t *extbctype, const int32_t *start_index,
+                                const int32_t *nrows_vars, const double *extbcvar1,
+                                const double *extbcvar2, double simtime, int32_t rank,
+                                int32_t total_process, void *stream);
+/* src/kernels_wetdry.h:24-85 Kernels::wet_dry (maxh may be NULL) */
+int tr_wet_dry_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, const double *h,
+                   double *qx, double *qy, const double *dem, double *maxh, double hextra,
+                   int32_t mpi_tasks, void *stream);
+
+/* ---- device-resident simulation ------------------------------------------ */
+typedef struct tr_sim tr_sim;
+
+typedef struct tr_sim_status {
+  double simtime;       /* simulated time reached */
+  double dt;            /* dt of the last step taken */
+  int64_t steps;        /* steps taken since creation */
+  int32_t print_id;     /* print intervals completed */
+  int32_t finished;     /* simtime >= sim_duration */
+  double volume;        /* FP64 water volume over interior cells (m^3) */
+  double volume_in;     /* FP64 integrated rain + inflow volume (m^3) */
+} tr_sim_status;
+
+/* Allocate device state for `view` in the given form and upload it. */
+int tr_sim_create(const tr_case_view *view, int32_t form, tr_sim **out);
+/* Advance until simtime >= t_stop (t_stop <= 0: the case's sim_duration) or
+ * `max_steps` more steps (max_steps < 0: unlimited), exactly as the
+ * reference's loop would (dt clamped to print times).  Blocking. */
+int tr_sim_run(tr_sim *s, double t_stop, int64_t max_steps, tr_sim_status *st);
+/* Enqueue exactly `nsteps` steps on the sim's stream without waiting (fused
+ * forms; steps past the end of the run are skipped on the device). */
+int tr_sim_enqueue(tr_sim *s, int64_t nsteps);
+int tr_sim_sync(tr_sim *s, tr_sim_status *st);
+/* Copy the current state to host (padded, FP64). Any pointer may be NULL. */
+int tr_sim_download(tr_sim *s, double *h, double *qx, double *qy, double *maxh);
+/* The hipStream_t the simulation runs on. */
+void *tr_sim_stream(tr_sim *s);
+/* Bytes of device memory held. */
+int64_t tr_sim_device_bytes(const tr_sim *s);
+void tr_sim_destroy(tr_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRITON_AMD_H */

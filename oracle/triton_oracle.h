@@ -1,0 +1,130 @@
+/*
+ * triton_oracle.h -- CPU FP64 restatement of TRITON's explicit finite-volume
+ * timestep.  TEST INFRASTRUCTURE ONLY: this library is the checker that the
+ * HIP path is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product never links it.
+ *
+ * The kernels are restated unfused, with the reference's array set, launch
+ * ranges and floating-point operation order, so that results are bit-for-bit
+ * those of the reference's FP64 Kokkos Serial/OpenMP build (every kernel is
+ * per-cell independent and the only reduction is an exact min).  Compile with
+ * -ffp-contract=off; sqrt is IEEE, cbrt comes from the host libm.
+ *
+ * Layout: row-major rows x cols arrays that include the one-cell ghost ring
+ * (rows = nrows_dem + 2, cols = ncols_dem + 2); row 0 is the north edge.
+ *
+ * Pinning (how each function is checked, see tests/test_oracle_*.py):
+ *   orc_compute_dt_and_sqrt, orc_find_min_dt, orc_flux_x, orc_flux_y,
+ *   orc_update_cells (no friction), orc_wet_dry, orc_run loop/dt logic:
+ *       reference fixtures test/reference/{circular,paraboloid}, plus
+ *       lake-at-rest, mirror-symmetry and mass-conservation invariants.
+ *   orc_update_cells (friction), orc_update_runoff, orc_compute_flow,
+ *   orc_compute_extbc_values (type 3): reference fixture test/reference/allatoona.
+ *   orc_open_bc_west_east / orc_open_bc_north_south: no reference fixture;
+ *       pinned by a uniform-flow invariant (exact steady state under
+ *       transmissive ghosts) and D4 symmetry of a centred dam break.
+ *   orc_compute_extbc_values types 0/1/2: parity unpinned (no fixture, no
+ *       closed form beyond the defining formula).
+ */
+#ifndef TRITON_ORACLE_H
+#define TRITON_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Everything the timestep needs, as the reference's init leaves it
+ * (src/triton_init.h:195-355, 723-830, 913-1055, 1299-1357). */
+typedef struct orc_case {
+  int rows, cols;             /* padded dims */
+  double dx;                  /* cell size */
+  double courant, hextra;
+  double *h, *qx, *qy;        /* state, in/out [rows*cols] */
+  const double *dem;          /* bed with ghost ring (walls 1e6 or copies) */
+  const double *nman;         /* squared Manning n [rows*cols] */
+  double *maxh;               /* running max depth or NULL */
+  /* zonal runoff (src/kernels.h:102-152) */
+  int num_runoffs, roff_nrows;
+  const double *roff_time;    /* [roff_nrows] seconds */
+  const double *roff_rate;    /* [num_runoffs*roff_nrows] m/s, zone-major */
+  const int *runoff_id;       /* [rows*cols] */
+  /* point inflows (src/kernels.h:50-84) */
+  int num_src, hyg_nrows;
+  const double *hyg_time;     /* [hyg_nrows] seconds */
+  const double *hyg_val;      /* [hyg_nrows*num_src] m^3/s, time-major */
+  const int *src_pos;         /* [num_src] flat cell index */
+  /* external boundary cells (src/kernels.h:236-424) */
+  int num_extbc_cells;
+  const int *bc_index, *bc_type, *bc_start, *bc_nrows;
+  const double *bc_var1, *bc_var2;
+  int open_boundaries;
+  /* time control (src/triton_simulate.h:12-226) */
+  double sim_start, sim_duration, print_interval, print_observation;
+  int time_series_flag, checkpoint_id;
+} orc_case;
+
+typedef struct orc_result {
+  long long steps;            /* iterations executed */
+  double simtime;             /* final simulated time */
+  double last_dt;             /* dt of the last step */
+  int print_id;               /* prints reached */
+  double init_dt;             /* all-dry fallback dt used by the run */
+} orc_result;
+
+/* Kernel restatements (argument meaning as in the reference). */
+void orc_compute_dt_and_sqrt(int size, double dx, const double *qx, const double *qy,
+                             const double *h, double *sqrth, double *dtarr,
+                             double cn, double hextra);
+double orc_find_min_dt(int size, const double *dtarr);
+void orc_flux_x(int rows, int cols, double dx, double dt, const double *h,
+                const double *qx, const double *qy, const double *dem,
+                const double *sqrth, double *rhsh0, double *rhsh1, double *rhsqx0,
+                double *rhsqx1, double *rhsqy0, double *rhsqy1, double hextra);
+void orc_flux_y(int rows, int cols, double dx, double dt, const double *h,
+                const double *qx, const double *qy, const double *dem,
+                const double *sqrth, double *rhsh0, double *rhsh1, double *rhsqx0,
+                double *rhsqx1, double *rhsqy0, double *rhsqy1, double hextra);
+void orc_update_cells(int rows, int cols, double dt, double *h, double *qx, double *qy,
+                      const double *nman, const double *rhsh0, const double *rhsh1,
+                      const double *rhsqx0, const double *rhsqx1, const double *rhsqy0,
+                      const double *rhsqy1, double hextra);
+void orc_update_runoff(int rows, int cols, double dt, const int *runoff_id,
+                       int index_row_runoff, int n_rows_runoff, const double *rate,
+                       double *h, double *qx, double *qy, double hextra);
+void orc_compute_flow(int num_src, const double *hyg_time, const double *hyg_val,
+                      double dx, double dt, double simtime, int idx_low, int idx_high,
+                      double *h, const int *src_pos);
+void orc_open_bc_west_east(int rows, int cols, double *h, double *qx, double *qy);
+void orc_open_bc_north_south(int rows, int cols, double *h, double *qx, double *qy);
+void orc_compute_extbc_values(int ncells, int rows, int cols, double dt, double *h,
+                              double *qx, double *qy, const double *dem,
+                              const double *nman, const int *bc_index,
+                              const int *bc_type, const int *bc_start,
+                              const int *bc_nrows, const double *var1,
+                              const double *var2, double simtime);
+void orc_wet_dry(int rows, int cols, const double *h, double *qx, double *qy,
+                 const double *dem, double *maxh, double hextra);
+
+/* Per-face solver, exposed for unit tests: fills d1 (west/south cell) and
+ * d2 (east/north cell) fluctuations before the dt/dx scaling; returns 0 when
+ * both cells are dry (h1<=0 && h2<=0) and nothing is computed. */
+int orc_face(double h1, double h2, double qx1, double qx2, double qy1, double qy2,
+             double z1, double z2, double sqh1, double sqh2, double nx, double ny,
+             double dx, double dt, double hextra, double d1[3], double d2[3]);
+
+/* The reference's simulate() loop (src/triton_simulate.h:54-134) from
+ * c->sim_start while simtime < min(t_stop, sim_duration) (t_stop <= 0: no
+ * extra limit), or until `max_steps` iterations when max_steps >= 0.  A
+ * print raster of the reference is the state when simtime first reaches
+ * print_interval*(id+1); pass that product as t_stop to stop there.  State
+ * arrays in `c` are advanced in place.  Returns 0 on success, nonzero on an
+ * allocation or index error. */
+int orc_run(const orc_case *c, double t_stop, long long max_steps, orc_result *res);
+
+/* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

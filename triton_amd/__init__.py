@@ -1,0 +1,428 @@
+"""triton_amd -- MI355X-native TRITON timestep (Python side of the C ABI).
+
+The compute path is libtriton_amd.so (hand-written HIP for gfx950 behind the
+C ABI in include/triton_amd.h).  This module only loads it with ctypes and
+wraps the case / simulation objects; there is no CPU fallback: when the
+library is missing every entry point raises.
+
+Mirrors of the reference's host interface (src/triton.h:55-61):
+  Triton(cfg).initialize() / .simulate()   -> Triton::triton<T>
+  Case.load(cfg)                           -> read_configuration + input loaders
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "_lib" / "libtriton_amd.so"
+
+FORM_F64_UNFUSED = 0
+FORM_F64_FUSED = 1
+FORM_F32_FUSED = 2
+
+TR_OK = 0
+_ERRNAMES = {1: "invalid argument", 2: "input error", 3: "HIP error", 4: "out of memory",
+             5: "bad state"}
+
+
+class TritonError(RuntimeError):
+    """Raised for any nonzero status from the C ABI."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{_ERRNAMES.get(code, code)}] {msg}")
+        self.code = code
+
+
+_p_d = C.POINTER(C.c_double)
+_p_i = C.POINTER(C.c_int32)
+
+
+class CaseView(C.Structure):
+    """ctypes image of tr_case_view (include/triton_amd.h)."""
+
+    _fields_ = [
+        ("rows", C.c_int32), ("cols", C.c_int32),
+        ("dx", C.c_double), ("xll", C.c_double), ("yll", C.c_double),
+        ("courant", C.c_double), ("hextra", C.c_double),
+        ("h", _p_d), ("qx", _p_d), ("qy", _p_d), ("dem", _p_d), ("nman2", _p_d),
+        ("maxh", _p_d),
+        ("runoff_id", _p_i),
+        ("num_runoffs", C.c_int32), ("roff_nrows", C.c_int32), ("roff_row0", C.c_int32),
+        ("roff_time", _p_d), ("roff_rate", _p_d),
+        ("num_src", C.c_int32), ("hyg_nrows", C.c_int32),
+        ("hyg_time", _p_d), ("hyg_val", _p_d), ("src_pos", _p_i),
+        ("num_extbc_cells", C.c_int32),
+        ("bc_index", _p_i), ("bc_type", _p_i), ("bc_start", _p_i), ("bc_nrows", _p_i),
+        ("bc_nvars", C.c_int32),
+        ("bc_var1", _p_d), ("bc_var2", _p_d),
+        ("open_boundaries", C.c_int32),
+        ("sim_start", C.c_double), ("sim_duration", C.c_double),
+        ("print_interval", C.c_double), ("print_observation", C.c_double),
+        ("init_dt", C.c_double),
+        ("time_series_flag", C.c_int32), ("checkpoint_id", C.c_int32), ("it_print", C.c_int32),
+    ]
+
+
+class SimStatus(C.Structure):
+    _fields_ = [
+        ("simtime", C.c_double), ("dt", C.c_double), ("steps", C.c_int64),
+        ("print_id", C.c_int32), ("finished", C.c_int32),
+        ("volume", C.c_double), ("volume_in", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Load libtriton_amd.so; raises when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (there is no CPU fallback)")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "tr_last_error": (C.c_char_p, []),
+        "tr_version": (C.c_char_p, []),
+        "tr_case_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "tr_case_view_get": (C.c_int, [vp, C.POINTER(CaseView)]),
+        "tr_case_free": (None, [vp]),
+        "tr_write_output_ascii": (C.c_int, [C.c_char_p, i32, i32, _p_d]),
+        "tr_write_output_binary": (C.c_int, [C.c_char_p, i32, i32, _p_d]),
+        "tr_sim_create": (C.c_int, [C.POINTER(CaseView), i32, C.POINTER(vp)]),
+        "tr_sim_run": (C.c_int, [vp, f64, i64, C.POINTER(SimStatus)]),
+        "tr_sim_enqueue": (C.c_int, [vp, i64]),
+        "tr_sim_sync": (C.c_int, [vp, C.POINTER(SimStatus)]),
+        "tr_sim_download": (C.c_int, [vp, _p_d, _p_d, _p_d, _p_d]),
+        "tr_sim_stream": (vp, [vp]),
+        "tr_sim_device_bytes": (i64, [vp]),
+        "tr_sim_destroy": (None, [vp]),
+        "tr_compute_dt_and_sqrt_f64": (C.c_int, [i32, f64, vp, vp, vp, vp, vp, f64, f64, vp]),
+        "tr_find_min_dt_f64": (C.c_int, [i32, vp, vp, _p_d, vp]),
+        "tr_flux_x_f64": (C.c_int, [i32, i32, i32, f64, f64] + [vp] * 11 + [f64, vp]),
+        "tr_flux_y_f64": (C.c_int, [i32, i32, i32, f64, f64] + [vp] * 11 + [f64, vp]),
+        "tr_update_cells_f64": (C.c_int, [i32, i32, i32, f64] + [vp] * 11 + [f64, vp]),
+        "tr_update_runoff_f64": (C.c_int, [i32, i32, i32, f64, vp, i32, i32, vp, vp, vp, vp, f64, vp]),
+        "tr_compute_flow_f64": (C.c_int, [i32, vp, vp, f64, f64, f64, i32, i32, vp, vp, vp]),
+        "tr_open_bc_west_east_f64": (C.c_int, [i32, i32, i32, vp, vp, vp, vp]),
+        "tr_open_bc_north_south_f64": (C.c_int, [i32, i32, i32, vp, vp, vp, i32, i32, vp]),
+        "tr_compute_extbc_values_f64": (C.c_int, [i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp,
+                                                  vp, vp, vp, vp, f64, i32, i32, vp]),
+        "tr_wet_dry_f64": (C.c_int, [i32, i32, i32, f64, vp, vp, vp, vp, vp, f64, i32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != TR_OK:
+        raise TritonError(rc, lib().tr_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Case:
+    """A loaded or synthesized case: padded FP64 arrays plus forcing tables.
+
+    Arrays keep the reference layout: rows x cols with the ghost ring, row 0
+    north (src/constants.h:82-112)."""
+
+    ARRAYS_F = ("h", "qx", "qy", "dem", "nman2", "maxh")
+
+    def __init__(self):
+        self._handle = None
+        self.arrays: dict[str, np.ndarray] = {}
+        self.scalars: dict = {}
+
+    # -- construction -----------------------------------------------------
+    @classmethod
+    def load(cls, cfg_path: str | os.PathLike, base_dir: str | os.PathLike | None = None) -> "Case":
+        """Parse a .cfg and read every input it names (reference formats)."""
+        L = lib()
+        h = C.c_void_p()
+        check(L.tr_case_load(str(cfg_path).encode(),
+                             None if base_dir is None else str(base_dir).encode(), C.byref(h)))
+        self = cls()
+        self._handle = h
+        v = CaseView()
+        check(L.tr_case_view_get(h, C.byref(v)))
+        n = v.rows * v.cols
+        for k in cls.ARRAYS_F:
+            self.arrays[k] = np.ctypeslib.as_array(getattr(v, k), shape=(v.rows, v.cols)).copy()
+        if v.num_runoffs > 0:
+            self.arrays["runoff_id"] = np.ctypeslib.as_array(v.runoff_id, shape=(v.rows, v.cols)).copy()
+            self.arrays["roff_time"] = np.ctypeslib.as_array(v.roff_time, shape=(v.roff_nrows,)).copy()
+            self.arrays["roff_rate"] = np.ctypeslib.as_array(
+                v.roff_rate, shape=(v.num_runoffs, v.roff_nrows)).copy()
+        if v.num_src > 0:
+            self.arrays["hyg_time"] = np.ctypeslib.as_array(v.hyg_time, shape=(v.hyg_nrows,)).copy()
+            self.arrays["hyg_val"] = np.ctypeslib.as_array(
+                v.hyg_val, shape=(v.hyg_nrows, v.num_src)).copy()
+            self.arrays["src_pos"] = np.ctypeslib.as_array(v.src_pos, shape=(v.num_src,)).copy()
+        if v.num_extbc_cells > 0:
+            k = v.num_extbc_cells
+            for nm in ("bc_index", "bc_type", "bc_start", "bc_nrows"):
+                self.arrays[nm] = np.ctypeslib.as_array(getattr(v, nm), shape=(k,)).copy()
+            self.arrays["bc_var1"] = np.ctypeslib.as_array(v.bc_var1, shape=(v.bc_nvars,)).copy()
+            self.arrays["bc_var2"] = np.ctypeslib.as_array(v.bc_var2, shape=(v.bc_nvars,)).copy()
+        for k in ("dx", "xll", "yll", "courant", "hextra", "roff_row0", "open_boundaries",
+                  "sim_start", "sim_duration", "print_interval", "print_observation",
+                  "init_dt", "time_series_flag", "checkpoint_id", "it_print"):
+            self.scalars[k] = getattr(v, k)
+        assert n == self.arrays["h"].size
+        L.tr_case_free(h)
+        self._handle = None
+        return self
+
+    @classmethod
+    def from_arrays(cls, *, dem, h, qx=None, qy=None, nman=None, dx=1.0, courant=0.5,
+                    hextra=1e-3, sim_duration=1.0, print_interval=None, open_boundaries=0,
+                    walls=True, runoff=None, sources=None, track_maxh=False,
+                    time_series_flag=0, print_observation=None) -> "Case":
+        """Build a case from unpadded (nrows x ncols) arrays the way the
+        reference's init pads them (src/triton_init.h:195-355): ghost ring,
+        walls (bed 1e6) or ghost copies, squared Manning n.
+
+        runoff: dict(time_s=[...], rate_ms=[[zone rows]], zone=int array) or None.
+        sources: dict(rows=[...], cols=[...] (unpadded), time_s=[...], q=[[t x src]]) or None.
+        """
+        dem = np.asarray(dem, np.float64)
+        nr, nc = dem.shape
+        rows, cols = nr + 2, nc + 2
+
+        def pad(a, fill=0.0):
+            out = np.full((rows, cols), fill, np.float64)
+            out[1:-1, 1:-1] = a
+            return out
+
+        def ghost_copy(a):
+            a[:, 0] = a[:, 1]
+            a[:, -1] = a[:, -2]
+            a[0, :] = a[1, :]
+            a[-1, :] = a[-2, :]
+            return a
+
+        self = cls()
+        z = pad(dem)
+        if open_boundaries:
+            ghost_copy(z)
+        else:
+            z[0, :] = z[-1, :] = 1e6
+            z[:, 0] = z[:, -1] = 1e6
+        n = np.zeros((nr, nc)) if nman is None else np.broadcast_to(np.asarray(nman, np.float64), (nr, nc))
+        n2 = ghost_copy(pad(n)) ** 2
+        st = []
+        for a in (h, qx, qy):
+            a = np.zeros((nr, nc)) if a is None else np.asarray(a, np.float64)
+            p = pad(a)
+            if open_boundaries:
+                ghost_copy(p)
+            st.append(p)
+        self.arrays.update(h=st[0], qx=st[1], qy=st[2], dem=z, nman2=n2)
+        if track_maxh:
+            self.arrays["maxh"] = st[0].copy()
+        pi = float(print_interval if print_interval is not None else sim_duration)
+        po = float(print_observation if print_observation is not None else pi)
+        init_dt = min(po, pi) if time_series_flag else pi
+        if runoff is not None:
+            rt = np.asarray(runoff["time_s"], np.float64)
+            rr = np.atleast_2d(np.asarray(runoff["rate_ms"], np.float64))
+            zone = np.zeros((nr, nc), np.int32) if runoff.get("zone") is None else np.asarray(runoff["zone"], np.int32)
+            self.arrays["roff_time"] = rt
+            self.arrays["roff_rate"] = np.ascontiguousarray(rr)
+            rid = np.zeros((rows, cols), np.int32)
+            rid[1:-1, 1:-1] = zone
+            self.arrays["runoff_id"] = rid
+            init_dt = min(init_dt, rt[1] - rt[0])
+            self.scalars["roff_row0"] = 0
+        if sources is not None:
+            ht = np.asarray(sources["time_s"], np.float64)
+            hv = np.asarray(sources["q"], np.float64).reshape(len(ht), -1)
+            pos = (np.asarray(sources["rows"]) + 1) * cols + np.asarray(sources["cols"]) + 1
+            self.arrays["hyg_time"] = ht
+            self.arrays["hyg_val"] = np.ascontiguousarray(hv)
+            self.arrays["src_pos"] = pos.astype(np.int32)
+            init_dt = min(init_dt, ht[1] - ht[0])
+        self.scalars.update(dx=float(dx), xll=0.0, yll=0.0, courant=float(courant),
+                            hextra=float(hextra), open_boundaries=int(open_boundaries),
+                            sim_start=0.0, sim_duration=float(sim_duration), print_interval=pi,
+                            print_observation=po, init_dt=init_dt * 0.01,
+                            time_series_flag=int(time_series_flag), checkpoint_id=0,
+                            it_print=10000)
+        self.scalars.setdefault("roff_row0", 0)
+        return self
+
+    # -- views ------------------------------------------------------------
+    @property
+    def rows(self) -> int:
+        return self.arrays["h"].shape[0]
+
+    @property
+    def cols(self) -> int:
+        return self.arrays["h"].shape[1]
+
+    def view(self) -> CaseView:
+        """A tr_case_view pointing into this object's arrays (keep self alive)."""
+        a, s = self.arrays, self.scalars
+        for k in list(a):
+            want = np.int32 if a[k].dtype.kind == "i" else np.float64
+            a[k] = np.ascontiguousarray(a[k], dtype=want)
+        v = CaseView()
+        v.rows, v.cols = self.rows, self.cols
+        for k in ("dx", "xll", "yll", "courant", "hextra", "sim_start", "sim_duration",
+                  "print_interval", "print_observation", "init_dt"):
+            setattr(v, k, float(s.get(k, 0.0)))
+        for k in ("open_boundaries", "time_series_flag", "checkpoint_id", "it_print", "roff_row0"):
+            setattr(v, k, int(s.get(k, 0)))
+        for k in self.ARRAYS_F:
+            setattr(v, k, _ptr(a.get(k), C.c_double))
+        if "roff_time" in a:
+            v.num_runoffs = a["roff_rate"].shape[0]
+            v.roff_nrows = a["roff_time"].shape[0]
+            v.roff_time = _ptr(a["roff_time"], C.c_double)
+            v.roff_rate = _ptr(a["roff_rate"], C.c_double)
+            v.runoff_id = _ptr(a.get("runoff_id"), C.c_int32)
+        if "src_pos" in a:
+            v.num_src = a["src_pos"].shape[0]
+            v.hyg_nrows = a["hyg_time"].shape[0]
+            v.hyg_time = _ptr(a["hyg_time"], C.c_double)
+            v.hyg_val = _ptr(a["hyg_val"], C.c_double)
+            v.src_pos = _ptr(a["src_pos"], C.c_int32)
+        if "bc_index" in a:
+            v.num_extbc_cells = a["bc_index"].shape[0]
+            for nm in ("bc_index", "bc_type", "bc_start", "bc_nrows"):
+                setattr(v, nm, _ptr(a[nm], C.c_int32))
+            v.bc_nvars = a["bc_var1"].shape[0]
+            v.bc_var1 = _ptr(a["bc_var1"], C.c_double)
+            v.bc_var2 = _ptr(a["bc_var2"], C.c_double)
+        return v
+
+    def copy(self) -> "Case":
+        c = Case()
+        c.arrays = {k: v.copy() for k, v in self.arrays.items()}
+        c.scalars = dict(self.scalars)
+        return c
+
+
+class Simulation:
+    """A device-resident simulation of one case (tr_sim_*)."""
+
+    def __init__(self, case: Case, form: int = FORM_F64_FUSED):
+        self.case = case
+        self.form = form
+        self._view = case.view()
+        h = C.c_void_p()
+        check(lib().tr_sim_create(C.byref(self._view), int(form), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().tr_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, t_stop: float = 0.0, max_steps: int = -1) -> dict:
+        st = SimStatus()
+        check(lib().tr_sim_run(self._h, float(t_stop), int(max_steps), C.byref(st)))
+        return st.as_dict()
+
+    def enqueue(self, nsteps: int) -> None:
+        check(lib().tr_sim_enqueue(self._h, int(nsteps)))
+
+    def sync(self) -> dict:
+        st = SimStatus()
+        check(lib().tr_sim_sync(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def state(self) -> dict[str, np.ndarray]:
+        shape = (self.case.rows, self.case.cols)
+        out = {k: np.empty(shape, np.float64) for k in ("h", "qx", "qy", "maxh")}
+        check(lib().tr_sim_download(self._h, *(_ptr(out[k], C.c_double) for k in ("h", "qx", "qy", "maxh"))))
+        return out
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().tr_sim_device_bytes(self._h))
+
+
+class Triton:
+    """Mirror of Triton::triton<T> (src/triton.h:32-327) for one .cfg:
+    initialize() loads the inputs and uploads them, simulate() runs to
+    sim_duration and, like the reference, writes H/QX/QY rasters at each print
+    when an output folder is given."""
+
+    def __init__(self, cfg_path, base_dir=None, form: int = FORM_F64_FUSED):
+        self.cfg_path = cfg_path
+        self.base_dir = base_dir
+        self.form = form
+        self.case: Case | None = None
+        self.sim: Simulation | None = None
+
+    def initialize(self) -> None:
+        self.case = Case.load(self.cfg_path, self.base_dir)
+        self.sim = Simulation(self.case, self.form)
+
+    def simulate(self, output_dir=None, pattern="%s/%s/%s_%02d_%02d") -> dict:
+        if self.sim is None:
+            raise TritonError(5, "simulate() before initialize()")
+        s = self.case.scalars
+        while True:
+            st = self.sim.run(t_stop=min(s["sim_duration"], s["print_interval"] * (self.sim_print_id() + 1)))
+            if output_dir is not None:
+                self.write_output(output_dir, st["print_id"], pattern)
+            if st["finished"]:
+                return st
+
+    def sim_print_id(self) -> int:
+        return int(self.sim.sync()["print_id"])
+
+    def write_output(self, output_dir, print_id: int, pattern="%s/%s/%s_%02d_%02d") -> None:
+        """ASC and BIN rasters of H, QX, QY (src/output.h:512-709, :737-752)."""
+        stt = self.sim.state()
+        L = lib()
+        for sub, fn in (("asc", L.tr_write_output_ascii), ("bin", L.tr_write_output_binary)):
+            os.makedirs(os.path.join(output_dir, sub), exist_ok=True)
+            for name, key in (("H", "h"), ("QX", "qx"), ("QY", "qy")):
+                path = (pattern % (str(output_dir), sub, name, print_id, 0)) + ".out"
+                a = np.ascontiguousarray(stt[key])
+                check(fn(path.encode(), a.shape[0], a.shape[1], _ptr(a, C.c_double)))
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible (does not initialise torch)."""
+    try:
+        import torch  # noqa: WPS433 -- plumbing only
+        return bool(torch.cuda.is_available())
+    except Exception:
+        return False

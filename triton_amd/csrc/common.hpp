@@ -1,0 +1,64 @@
+// common.hpp -- constants, error plumbing and launch helpers shared by the
+// HIP translation units of libtriton_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/triton_amd.h"
+
+namespace tr {
+
+// Physical and numerical constants of the FP64 reference build
+// (src/constants.h:43-50,117-118).  SQRTG is the reference's rounded value,
+// not sqrt(9.81).
+constexpr double kG = 9.81;
+constexpr double kSqrtG = 3.132091953;
+constexpr double kEps12 = 1e-12;
+constexpr double kMaxDt = 1e10;
+
+constexpr int kBlock = 256;  // 1-D launches of the drop-in kernels
+
+// Thread-local error message behind tr_last_error().
+void set_error(const std::string& msg);
+void clear_error();
+
+inline int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return TR_ERR_HIP;
+}
+
+#define TR_HIP(call)                                           \
+  do {                                                         \
+    hipError_t tr_e_ = (call);                                 \
+    if (tr_e_ != hipSuccess) return ::tr::hip_fail(tr_e_, #call); \
+  } while (0)
+
+inline int launch_check(const char* name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, name);
+  return TR_OK;
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  return unsigned((n + block - 1) / block);
+}
+
+// Positive doubles order like their uint64 bit patterns, so an exact min of
+// per-cell dt values is an unsigned 64-bit atomic min.
+__host__ __device__ inline unsigned long long dbits(double v) {
+  union { double d; unsigned long long u; } x;
+  x.d = v;
+  return x.u;
+}
+__host__ __device__ inline double bitsd(unsigned long long u) {
+  union { double d; unsigned long long u; } x;
+  x.u = u;
+  return x.d;
+}
+
+}  // namespace tr

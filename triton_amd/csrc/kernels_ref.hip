@@ -1,0 +1,452 @@
+// kernels_ref.hip -- FP64 drop-in replacements for the reference's per-kernel
+// entry points (Kernels::* in src/kernels*.h), one thread per launch item
+// with the reference's launch range and active sub-range, so a host that
+// replays compute_new_state (src/triton_simulate.h:230-346) with these gets
+// the FP64 CPU results bit for bit.  Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "roe_f64.hpp"
+
+namespace tr {
+namespace {
+
+__global__ void k_compute_dt_and_sqrt(int size, double cn_dx, const double* __restrict__ qx,
+                                      const double* __restrict__ qy,
+                                      const double* __restrict__ h, double* __restrict__ sqrth,
+                                      double* __restrict__ out, double hextra) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  double hc = h[id];
+  double sq = fmax(sqrt(hc), 0.0);
+  sqrth[id] = sq;
+  out[id] = cell_dt_f64(hc, qx[id], qy[id], sq, cn_dx, hextra);
+}
+
+// Total-order uint64 key of a non-NaN double (negative values flip all bits,
+// non-negative ones set the sign bit) and its inverse.
+__device__ __forceinline__ unsigned long long order_key(double v) {
+  unsigned long long u = dbits(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_to_double(unsigned long long k) {
+  return bitsd((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+}
+
+// Wave64 min of uint64 keys, one atomic per wave.
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__global__ void k_init_min(unsigned long long* slot) { *slot = ~0ull; }
+
+__global__ void k_min_reduce(int size, const double* __restrict__ in,
+                             unsigned long long* __restrict__ slot) {
+  unsigned long long best = ~0ull;
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < size; id += gridDim.x * blockDim.x) {
+    double v = in[id];
+    // order-preserving uint64 key of a double; NaN never wins, as with the
+    // reference's `a < b ? a : b` form
+    unsigned long long b = (v == v) ? order_key(v) : ~0ull;
+    best = b < best ? b : best;
+  }
+  best = wave_min_u64(best);
+  if ((threadIdx.x & 63) == 0) atomicMin(slot, best);
+}
+
+__global__ void k_bits_to_double(const unsigned long long* slot, double* out) {
+  unsigned long long b = *slot;
+  *out = (b == ~0ull) ? INFINITY : key_to_double(b);
+}
+
+__global__ void k_flux_x(int size, int nrows, int ncols, double dx, double dt,
+                         const double* __restrict__ h, const double* __restrict__ qx,
+                         const double* __restrict__ qy, const double* __restrict__ dem,
+                         const double* __restrict__ sqrth, double* __restrict__ rhsh0,
+                         double* __restrict__ rhsh1, double* __restrict__ rhsqx0,
+                         double* __restrict__ rhsqx1, double* __restrict__ rhsqy0,
+                         double* __restrict__ rhsqy1, double hextra) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  int r = id / ncols, c = id % ncols;
+  if (r < 1 || r >= nrows - 1 || c >= ncols - 1) return;  // src/kernels_flux.h:62-71
+  int a = id, b = id + 1;
+  Face64 f;
+  if (roe_face_f64<1, 0>(h[a], h[b], qx[a], qx[b], qy[a], qy[b], dem[a], dem[b], sqrth[a],
+                         sqrth[b], dx, dt, hextra, f)) {
+    rhsh0[a] = dt * f.d1[0] / dx;
+    rhsqx0[a] = dt * f.d1[1] / dx;
+    rhsqy0[a] = dt * f.d1[2] / dx;
+    rhsh1[b] = dt * f.d2[0] / dx;
+    rhsqx1[b] = dt * f.d2[1] / dx;
+    rhsqy1[b] = dt * f.d2[2] / dx;
+  } else {
+    rhsh0[a] = 0.0; rhsqx0[a] = 0.0; rhsqy0[a] = 0.0;
+    rhsh1[b] = 0.0; rhsqx1[b] = 0.0; rhsqy1[b] = 0.0;
+  }
+}
+
+__global__ void k_flux_y(int size, int nrows, int ncols, double dx, double dt,
+                         const double* __restrict__ h, const double* __restrict__ qx,
+                         const double* __restrict__ qy, const double* __restrict__ dem,
+                         const double* __restrict__ sqrth, double* __restrict__ rhsh0,
+                         double* __restrict__ rhsh1, double* __restrict__ rhsqx0,
+                         double* __restrict__ rhsqx1, double* __restrict__ rhsqy0,
+                         double* __restrict__ rhsqy1, double hextra) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  int r = id / ncols, c = id % ncols;
+  if (r < 1 || c < 1 || c >= ncols - 1) return;  // src/kernels_flux.h:342-354
+  int a = id, b = id - ncols;
+  Face64 f;
+  if (roe_face_f64<0, 1>(h[a], h[b], qx[a], qx[b], qy[a], qy[b], dem[a], dem[b], sqrth[a],
+                         sqrth[b], dx, dt, hextra, f)) {
+    rhsh0[a] += dt * f.d1[0] / dx;
+    rhsqx0[a] += dt * f.d1[1] / dx;
+    rhsqy0[a] += dt * f.d1[2] / dx;
+    rhsh1[b] += dt * f.d2[0] / dx;
+    rhsqx1[b] += dt * f.d2[1] / dx;
+    rhsqy1[b] += dt * f.d2[2] / dx;
+  }
+}
+
+__global__ void k_update_cells(int size, int nrows, int ncols, double dt, double* __restrict__ h,
+                               double* __restrict__ qx, double* __restrict__ qy,
+                               const double* __restrict__ nman, const double* __restrict__ rhsh0,
+                               const double* __restrict__ rhsh1,
+                               const double* __restrict__ rhsqx0,
+                               const double* __restrict__ rhsqx1,
+                               const double* __restrict__ rhsqy0,
+                               const double* __restrict__ rhsqy1, double hextra) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  int r = id / ncols, c = id % ncols;
+  if (r < 1 || r >= nrows - 1 || c < 1 || c >= ncols - 1) return;
+  double ho, qxo, qyo;
+  update_cell_f64(h[id], qx[id], qy[id], nman[id], rhsh0[id], rhsh1[id], rhsqx0[id],
+                  rhsqx1[id], rhsqy0[id], rhsqy1[id], dt, hextra, ho, qxo, qyo);
+  h[id] = ho;
+  qx[id] = qxo;
+  qy[id] = qyo;
+}
+
+__global__ void k_update_runoff(int size, int nrows, int ncols, double dt,
+                                const int* __restrict__ rid, int row, int nrow,
+                                const double* __restrict__ rate, double* __restrict__ h,
+                                double* __restrict__ qx, double* __restrict__ qy, double hextra) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  int r = id / ncols, c = id % ncols;
+  if (r < 1 || r >= nrows - 1 || c < 1 || c >= ncols - 1) return;
+  double hc = h[id];
+  hc += rate[rid[id] * nrow + row] * dt;
+  if (hc < hextra) {
+    if (hc < 0.0) hc = 0.0;
+    qx[id] = 0.0;
+    qy[id] = 0.0;
+  }
+  h[id] = hc;
+}
+
+// One thread walks the sources in order: duplicates accumulate
+// deterministically where the reference's parallel update would race.
+__global__ void k_compute_flow(int nsrc, const double* __restrict__ ht,
+                               const double* __restrict__ hv, double dx, double dt, double simtime,
+                               int lo, int hi, double* __restrict__ h, const int* __restrict__ pos) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int s = 0; s < nsrc; s++) {
+    double flo = hv[lo * nsrc + s], fhi = hv[hi * nsrc + s];
+    double q;
+    if (lo == hi) q = flo;
+    else q = flo + (((fhi - flo) * (simtime - ht[lo])) / (ht[hi] - ht[lo]));
+    int cell = pos[s];
+    h[cell] = h[cell] + (q * dt) / (dx * dx);
+  }
+}
+
+__global__ void k_open_we(int size, int nrows, int ncols, double* __restrict__ h,
+                          double* __restrict__ qx, double* __restrict__ qy) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  int g, in;
+  if (id < nrows) { g = (id % nrows) * ncols; in = g + 1; }
+  else { g = (id % nrows) * ncols + ncols - 1; in = g - 1; }
+  h[g] = h[in]; qx[g] = qx[in]; qy[g] = qy[in];
+}
+
+__global__ void k_open_ns(int size, int nrows, int ncols, double* __restrict__ h,
+                          double* __restrict__ qx, double* __restrict__ qy, int rank, int nproc) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  if (rank > 0 && rank < nproc - 1) return;
+  if (rank == 0) {
+    int g = id, in = (id % ncols) + ncols;
+    h[g] = h[in]; qx[g] = qx[in]; qy[g] = qy[in];
+  }
+  if (rank == nproc - 1) {
+    int g = id + (nrows - 1) * ncols, in = (id % ncols) + (nrows - 2) * ncols;
+    h[g] = h[in]; qx[g] = qx[in]; qy[g] = qy[in];
+  }
+}
+
+// External boundary cells; one thread walks the list in order so a cell
+// listed by two segments gets the later segment's values deterministically.
+__global__ void k_extbc(int ncells, int nrows, int ncols, double dt, double* __restrict__ h,
+                        double* __restrict__ qx, double* __restrict__ qy,
+                        const double* __restrict__ dem, const double* __restrict__ nman,
+                        const int* __restrict__ idx, const int* __restrict__ type,
+                        const int* __restrict__ start, const int* __restrict__ nvars,
+                        const double* __restrict__ v1, const double* __restrict__ v2,
+                        double simtime, int rank, int nproc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int k = 0; k < ncells; k++) {
+    int ii = idx[k], bt = type[k], st = start[k];
+    int r = ii / ncols, c = ii % ncols;
+    bool top = r == 1 && rank == 0, btm = r == nrows - 2 && rank == nproc - 1;
+    bool lt = c == 1, rt = c == ncols - 2;
+    if (!(top || lt || rt || btm)) continue;
+    int ghost = ii, corner = -1;
+    double f0 = 1.0, f1 = 0.0, f2 = 0.0;
+    if (lt) { ghost = ii - 1; f1 = -1.0; f2 = 0.0; }
+    if (rt) { ghost = ii + 1; f1 = 1.0; f2 = 0.0; }
+    if (top) {
+      ghost = ii - ncols; f1 = 0.0; f2 = 1.0;
+      if (lt) { corner = ii - 1; f2 = 0.0; }
+      if (rt) { corner = ii + 1; f2 = 0.0; }
+    }
+    if (btm) {
+      ghost = ii + ncols; f1 = 0.0; f2 = -1.0;
+      if (lt) { corner = ii - 1; f2 = 0.0; }
+      if (rt) { corner = ii + 1; f2 = 0.0; }
+    }
+    double hc = h[ii], qxc = qx[ii], qyc = qy[ii];
+    if (bt == 0) {
+      h[ghost] = hc; qx[ghost] = qxc; qy[ghost] = qyc;
+      if (corner >= 0) { h[corner] = hc; qx[corner] = qxc; qy[corner] = qyc; }
+      continue;
+    }
+    if (bt == 1) {
+      double tm = 0.5 * (simtime + simtime + dt);
+      int lo = -999, hi = nvars[k] - 1;
+      if (tm < v1[st]) { lo = 0; hi = 0; }
+      else if (tm > v1[hi + st]) { lo = hi; }
+      else {
+        for (int i = 0; i < hi; i++)
+          if (v1[i + st] <= tm && v1[i + 1 + st] > tm) { lo = i; hi = i + 1; break; }
+        if (lo < 0) lo = hi;
+      }
+      double level;
+      if (lo == hi) level = v2[lo + st];
+      else level = v2[lo + st] + (((v2[hi + st] - v2[lo + st]) * (tm - v1[lo + st])) /
+                                  (v1[hi + st] - v1[lo + st]));
+      hc = fmax(level - dem[ii], 0.0);
+    } else {
+      double p = v1[st], vel = 0.0;
+      if (bt == 2) vel = sqrt(p * hc * cbrt(hc) / fmax(nman[ii], 1e-6));
+      if (bt == 3) vel = p * sqrt(kG * hc);
+      qxc = hc * vel;
+      qyc = hc * vel;
+    }
+    h[ghost] = hc * f0; qx[ghost] = qxc * f1; qy[ghost] = qyc * f2;
+    h[ii] = hc * f0; qx[ii] = qxc * f1; qy[ii] = qyc * f2;
+    if (corner >= 0) { h[corner] = hc * f0; qx[corner] = qxc * f1; qy[corner] = qyc * f2; }
+  }
+}
+
+__global__ void k_wet_dry(int size, int nrows, int ncols, const double* __restrict__ h,
+                          double* __restrict__ qx, double* __restrict__ qy,
+                          const double* __restrict__ dem, double* __restrict__ maxh,
+                          double hextra, int mpi_tasks) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= size) return;
+  int r = id / ncols, c = id % ncols;
+  if (r < 1 || r >= nrows - 1 || c < 1 || c >= ncols - 1) return;
+  double hc = h[id], zc = dem[id];
+  if (hc > hextra) {
+    int e = id + 1, w = id - 1, n = id - ncols, s = id + ncols;
+    if ((hc + zc < dem[e] && h[e] < kEps12) || (hc + zc < dem[w] && h[w] < kEps12)) qx[id] = 0.0;
+    bool zero_y;
+    if (r == 1 && mpi_tasks > 1) zero_y = hc + zc < dem[s] && h[s] < kEps12;
+    else if (r == nrows - 2 && mpi_tasks > 1) zero_y = hc + zc < dem[n] && h[n] < kEps12;
+    else zero_y = (hc + zc < dem[n] && h[n] < kEps12) || (hc + zc < dem[s] && h[s] < kEps12);
+    if (zero_y) qy[id] = 0.0;
+  }
+  if (maxh && hc > maxh[id]) maxh[id] = hc;
+}
+
+}  // namespace
+
+// Shared by the unfused simulation driver: min of `size` dt values into the
+// device slot (uint64 image), optionally also as a double.
+int min_dt_to_slot(int size, const double* dtarr, unsigned long long* slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_min, dim3(1), dim3(1), 0, s, slot);
+  unsigned g = grid_for(size);
+  if (g > 4096) g = 4096;
+  if (g == 0) g = 1;
+  hipLaunchKernelGGL(k_min_reduce, dim3(g), dim3(kBlock), 0, s, size, dtarr, slot);
+  return launch_check("find_min_dt");
+}
+
+}  // namespace tr
+
+using namespace tr;
+
+extern "C" {
+
+int tr_compute_dt_and_sqrt_f64(int32_t size, double dx, const double* qx, const double* qy,
+                               const double* h, double* sqrth, double* dt_out, double cn,
+                               double hextra, void* stream) {
+  if (size < 0 || (size > 0 && (!qx || !qy || !h || !sqrth || !dt_out))) {
+    set_error("tr_compute_dt_and_sqrt_f64: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (size == 0) return TR_OK;
+  hipLaunchKernelGGL(k_compute_dt_and_sqrt, dim3(grid_for(size)), dim3(kBlock), 0,
+                     as_stream(stream), size, cn * dx, qx, qy, h, sqrth, dt_out, hextra);
+  return launch_check("compute_dt_and_sqrt");
+}
+
+int tr_find_min_dt_f64(int32_t size, const double* dtarr, double* d_min, double* h_min,
+                       void* stream) {
+  if (size <= 0 || !dtarr || !d_min) {
+    set_error("tr_find_min_dt_f64: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  hipStream_t s = as_stream(stream);
+  // the uint64 reduction slot lives in the 8 bytes of d_min itself
+  auto* slot = reinterpret_cast<unsigned long long*>(d_min);
+  int rc = min_dt_to_slot(size, dtarr, slot, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bits_to_double, dim3(1), dim3(1), 0, s, slot, d_min);
+  rc = launch_check("find_min_dt");
+  if (rc) return rc;
+  if (h_min) {
+    TR_HIP(hipMemcpyAsync(h_min, d_min, sizeof(double), hipMemcpyDeviceToHost, s));
+    TR_HIP(hipStreamSynchronize(s));
+  }
+  return TR_OK;
+}
+
+#define TR_CHECK_GRID(fn)                                                          \
+  if (size < 0 || nrows < 3 || ncols < 3 || int64_t(nrows) * ncols < size) {        \
+    set_error(#fn ": size must cover at most nrows*ncols with nrows,ncols >= 3"); \
+    return TR_ERR_INVALID;                                                         \
+  }                                                                                \
+  if (size == 0) return TR_OK;
+
+int tr_flux_x_f64(int32_t size, int32_t nrows, int32_t ncols, double dx, double dt,
+                  const double* h, const double* qx, const double* qy, const double* dem,
+                  const double* sqrth, double* rhsh0, double* rhsh1, double* rhsqx0,
+                  double* rhsqx1, double* rhsqy0, double* rhsqy1, double hextra, void* stream) {
+  TR_CHECK_GRID(tr_flux_x_f64)
+  hipLaunchKernelGGL(k_flux_x, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream), size,
+                     nrows, ncols, dx, dt, h, qx, qy, dem, sqrth, rhsh0, rhsh1, rhsqx0, rhsqx1,
+                     rhsqy0, rhsqy1, hextra);
+  return launch_check("flux_x");
+}
+
+int tr_flux_y_f64(int32_t size, int32_t nrows, int32_t ncols, double dx, double dt,
+                  const double* h, const double* qx, const double* qy, const double* dem,
+                  const double* sqrth, double* rhsh0, double* rhsh1, double* rhsqx0,
+                  double* rhsqx1, double* rhsqy0, double* rhsqy1, double hextra, void* stream) {
+  TR_CHECK_GRID(tr_flux_y_f64)
+  hipLaunchKernelGGL(k_flux_y, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream), size,
+                     nrows, ncols, dx, dt, h, qx, qy, dem, sqrth, rhsh0, rhsh1, rhsqx0, rhsqx1,
+                     rhsqy0, rhsqy1, hextra);
+  return launch_check("flux_y");
+}
+
+int tr_update_cells_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, double* h,
+                        double* qx, double* qy, const double* dem, const double* nman,
+                        double* rhsh0, double* rhsh1, double* rhsqx0, double* rhsqx1,
+                        double* rhsqy0, double* rhsqy1, double hextra, void* stream) {
+  (void)dem;  // part of the reference signature, unused there too
+  TR_CHECK_GRID(tr_update_cells_f64)
+  hipLaunchKernelGGL(k_update_cells, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream),
+                     size, nrows, ncols, dt, h, qx, qy, nman, rhsh0, rhsh1, rhsqx0, rhsqx1,
+                     rhsqy0, rhsqy1, hextra);
+  return launch_check("update_cells");
+}
+
+int tr_update_runoff_f64(int32_t size, int32_t nrows, int32_t ncols, double dt,
+                         const int32_t* runoff_id, int32_t index_row_runoff,
+                         int32_t n_rows_runoff, const double* runoff_intensity, double* h,
+                         double* qx, double* qy, double hextra, void* stream) {
+  TR_CHECK_GRID(tr_update_runoff_f64)
+  hipLaunchKernelGGL(k_update_runoff, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream),
+                     size, nrows, ncols, dt, runoff_id, index_row_runoff, n_rows_runoff,
+                     runoff_intensity, h, qx, qy, hextra);
+  return launch_check("update_runoff");
+}
+
+int tr_compute_flow_f64(int32_t size, const double* hyg_time, const double* hyg_val, double dx,
+                        double dt, double simtime, int32_t idx_low, int32_t idx_high, double* h,
+                        const int32_t* pos, void* stream) {
+  if (size < 0 || idx_low < 0 || idx_high < 0) {
+    set_error("tr_compute_flow_f64: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (size == 0) return TR_OK;
+  hipLaunchKernelGGL(k_compute_flow, dim3(1), dim3(64), 0, as_stream(stream), size, hyg_time,
+                     hyg_val, dx, dt, simtime, idx_low, idx_high, h, pos);
+  return launch_check("compute_flow");
+}
+
+int tr_open_bc_west_east_f64(int32_t size, int32_t nrows, int32_t ncols, double* h, double* qx,
+                             double* qy, void* stream) {
+  if (size < 0 || size > 2 * nrows || nrows < 3 || ncols < 3) {
+    set_error("tr_open_bc_west_east_f64: size must be at most 2*nrows");
+    return TR_ERR_INVALID;
+  }
+  if (size == 0) return TR_OK;
+  hipLaunchKernelGGL(k_open_we, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream), size,
+                     nrows, ncols, h, qx, qy);
+  return launch_check("open_bc_west_east");
+}
+
+int tr_open_bc_north_south_f64(int32_t size, int32_t nrows, int32_t ncols, double* h,
+                               double* qx, double* qy, int32_t rank, int32_t total_process,
+                               void* stream) {
+  if (size < 0 || size > ncols || nrows < 3 || ncols < 3 || total_process < 1) {
+    set_error("tr_open_bc_north_south_f64: size must be at most ncols");
+    return TR_ERR_INVALID;
+  }
+  if (size == 0) return TR_OK;
+  hipLaunchKernelGGL(k_open_ns, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream), size,
+                     nrows, ncols, h, qx, qy, rank, total_process);
+  return launch_check("open_bc_north_south");
+}
+
+int tr_compute_extbc_values_f64(int32_t size, int32_t nrows, int32_t ncols, double dt,
+                                double* h, double* qx, double* qy, const double* dem,
+                                const double* nman, const int32_t* relative_index,
+                                const int32_t* extbctype, const int32_t* start_index,
+                                const int32_t* nrows_vars, const double* extbcvar1,
+                                const double* extbcvar2, double simtime, int32_t rank,
+                                int32_t total_process, void* stream) {
+  if (size < 0 || nrows < 3 || ncols < 3) {
+    set_error("tr_compute_extbc_values_f64: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (size == 0) return TR_OK;
+  hipLaunchKernelGGL(k_extbc, dim3(1), dim3(64), 0, as_stream(stream), size, nrows, ncols, dt, h,
+                     qx, qy, dem, nman, relative_index, extbctype, start_index, nrows_vars,
+                     extbcvar1, extbcvar2, simtime, rank, total_process);
+  return launch_check("compute_extbc_values");
+}
+
+int tr_wet_dry_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, const double* h,
+                   double* qx, double* qy, const double* dem, double* maxh, double hextra,
+                   int32_t mpi_tasks, void* stream) {
+  (void)dt;  // part of the reference signature, unused there too
+  TR_CHECK_GRID(tr_wet_dry_f64)
+  hipLaunchKernelGGL(k_wet_dry, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream), size,
+                     nrows, ncols, h, qx, qy, dem, maxh, hextra, mpi_tasks);
+  return launch_check("wet_dry");
+}
+
+}  // extern "C"

@@ -1,0 +1,629 @@
+// sim.hip -- device-resident simulation objects behind tr_sim_*.
+//
+// TR_FORM_F64_UNFUSED replays Triton::triton<T>::simulate()
+// (src/triton_simulate.h:54-159) with the per-kernel drop-in entry points and
+// the same host-side dt / print / forcing logic, reading dt back every step as
+// the reference does.  TR_FORM_F64_FUSED and TR_FORM_F32_FUSED launch one
+// fused kernel per step and keep dt, time and forcing indices on the device;
+// the host only looks at the control block between batches of launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "step.hpp"
+
+namespace tr {
+int min_dt_to_slot(int size, const double* dtarr, unsigned long long* slot, hipStream_t s);
+int launch_prime_f64(const StepParams<double>& p, hipStream_t s);
+int launch_step_f64(const StepParams<double>& p, hipStream_t s);
+int fused_grid_x(int cols);
+}  // namespace tr
+
+using namespace tr;
+
+struct tr_sim {
+  int form = 0;
+  tr_case_view v{};               // scalars; array pointers are not kept
+  std::vector<double> roff_time, hyg_time;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+  // state (FP64); two buffers for the fused form
+  double* h[2] = {nullptr, nullptr};
+  double* qx[2] = {nullptr, nullptr};
+  double* qy[2] = {nullptr, nullptr};
+  double *dem = nullptr, *nman2 = nullptr, *maxh = nullptr;
+  int32_t* rid = nullptr;
+  double *rrate = nullptr, *rtime = nullptr, *zone_area = nullptr;
+  double *htime = nullptr, *hval = nullptr;
+  int32_t* srcpos = nullptr;
+  int32_t *src_row = nullptr, *src_col = nullptr, *src_id = nullptr;
+  double* src_add = nullptr;
+  int32_t *bc_index = nullptr, *bc_type = nullptr, *bc_start = nullptr, *bc_nrows = nullptr;
+  double *bc_v1 = nullptr, *bc_v2 = nullptr;
+  bool friction = false;
+  // unfused scratch
+  double *sqrth = nullptr, *dtarr = nullptr, *rhs[6] = {};
+  double* dmin = nullptr;
+  double* h_pinned = nullptr;
+  // unfused host loop state (src/triton_simulate.h:41-53)
+  double simtime = 0, last_dt = 0;
+  int print_id = 0, roff_row = 0, idx_low = 0;
+  int64_t steps = 0;
+  double vol_in = 0;
+  // fused control
+  StepCtrl* ctrl = nullptr;
+  StepCtrl* ctrl_host = nullptr;  // pinned mirror
+  int64_t launched = 0;            // == completed steps between API calls
+  int strip_rows = 64;
+  // volume reduction
+  double* vpart = nullptr;
+  double* vsum = nullptr;
+
+  ~tr_sim() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : allocs) hipFree(p);
+    if (ctrl_host) hipHostFree(ctrl_host);
+    if (h_pinned) hipHostFree(h_pinned);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc(T*& p, size_t n) {
+    p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+      return TR_ERR_NOMEM;
+    }
+    allocs.push_back(p);
+    bytes += int64_t(n * sizeof(T));
+    return TR_OK;
+  }
+  template <class T>
+  int upload(T*& p, const T* src, size_t n) {
+    int rc = alloc(p, n);
+    if (rc) return rc;
+    if (n && src) TR_HIP(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, stream));
+    return TR_OK;
+  }
+};
+
+namespace {
+
+#define RC(x)            \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
+__global__ void k_volume_part(const double* __restrict__ h, int rows, int cols, double area,
+                              double* __restrict__ part) {
+  __shared__ double sm[256];
+  const int n_int = (rows - 2) * (cols - 2);
+  const int per = (n_int + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(lo + per, n_int);
+  double acc = 0.0;
+  for (int k = lo + int(threadIdx.x); k < hi; k += blockDim.x) {
+    int r = k / (cols - 2) + 1, c = k % (cols - 2) + 1;
+    acc += h[(long long)r * cols + c];
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0] * area;
+}
+
+__global__ void k_volume_sum(const double* __restrict__ part, int n, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += part[i];
+  *out = s;
+}
+
+constexpr int kVolBlocks = 512;
+
+// Deterministic FP64 volume of the interior (fixed partition, fixed order).
+int volume_f64(tr_sim* s, const double* h, double* out) {
+  hipLaunchKernelGGL(k_volume_part, dim3(kVolBlocks), dim3(256), 0, s->stream, h, s->v.rows,
+                     s->v.cols, s->v.dx * s->v.dx, s->vpart);
+  hipLaunchKernelGGL(k_volume_sum, dim3(1), dim3(64), 0, s->stream, s->vpart, kVolBlocks,
+                     s->vsum);
+  RC(launch_check("volume"));
+  TR_HIP(hipMemcpyAsync(s->h_pinned, s->vsum, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  TR_HIP(hipStreamSynchronize(s->stream));
+  *out = *s->h_pinned;
+  return TR_OK;
+}
+
+int check_view(const tr_case_view* v) {
+  if (!v || v->rows < 3 || v->cols < 3 || !v->h || !v->qx || !v->qy || !v->dem || !v->nman2) {
+    set_error("tr_sim_create: view needs rows,cols >= 3 and h, qx, qy, dem, nman2");
+    return TR_ERR_INVALID;
+  }
+  if (int64_t(v->rows) * v->cols > int64_t(INT32_MAX)) {
+    set_error("tr_sim_create: grid exceeds 2^31 cells");
+    return TR_ERR_INVALID;
+  }
+  if (!(v->dx > 0) || !(v->courant > 0) || !(v->print_interval > 0)) {
+    set_error("tr_sim_create: dx, courant and print_interval must be positive");
+    return TR_ERR_INVALID;
+  }
+  if (v->num_runoffs > 0 && (v->roff_nrows < 2 || !v->roff_time || !v->roff_rate)) {
+    set_error("tr_sim_create: runoff needs at least two rows");
+    return TR_ERR_INVALID;
+  }
+  if (v->num_src > 0 && (v->hyg_nrows < 2 || !v->hyg_time || !v->hyg_val || !v->src_pos)) {
+    set_error("tr_sim_create: sources need a hydrograph with at least two rows");
+    return TR_ERR_INVALID;
+  }
+  const int64_t n = int64_t(v->rows) * v->cols;
+  for (int i = 0; i < v->num_src; i++) {
+    int r = v->src_pos[i] / v->cols, c = v->src_pos[i] % v->cols;
+    if (v->src_pos[i] < 0 || v->src_pos[i] >= n || r < 1 || r > v->rows - 2 || c < 1 || c > v->cols - 2) {
+      set_error("tr_sim_create: source cell outside the interior");
+      return TR_ERR_INVALID;
+    }
+  }
+  if (v->num_runoffs > 0 && v->runoff_id) {
+    for (int r = 1; r < v->rows - 1; r++)
+      for (int c = 1; c < v->cols - 1; c++) {
+        int z = v->runoff_id[int64_t(r) * v->cols + c];
+        if (z < 0 || z >= v->num_runoffs) {
+          set_error("tr_sim_create: runoff zone id out of range");
+          return TR_ERR_INVALID;
+        }
+      }
+  }
+  return TR_OK;
+}
+
+// ---- unfused form: the reference's loop, kernel by kernel ----------------
+int step_unfused(tr_sim* s) {
+  const tr_case_view& v = s->v;
+  const int rows = v.rows, cols = v.cols, n = rows * cols;
+  hipStream_t st = s->stream;
+  void* sp = st;
+  double* h = s->h[0];
+  double* qx = s->qx[0];
+  double* qy = s->qy[0];
+  // compute_local_dt: per-cell dt + blocking min (src/triton_simulate.h:191-198)
+  RC(tr_compute_dt_and_sqrt_f64(n, v.dx, qx, qy, h, s->sqrth, s->dtarr, v.courant, v.hextra, sp));
+  double dt = 0;
+  RC(tr_find_min_dt_f64(n, s->dtarr, s->dmin, s->h_pinned, sp));
+  dt = *s->h_pinned;
+  // compute_global_dt (:202-226)
+  if (dt >= kMaxDt - 1.0) dt = v.init_dt;
+  if (s->simtime + dt > v.print_interval * (s->print_id + 1))
+    dt = v.print_interval * (s->print_id + 1) - s->simtime;
+  // compute_new_state (:230-346)
+  RC(tr_flux_x_f64(n, rows, cols, v.dx, dt, h, qx, qy, s->dem, s->sqrth, s->rhs[0], s->rhs[1],
+                   s->rhs[2], s->rhs[3], s->rhs[4], s->rhs[5], v.hextra, sp));
+  RC(tr_flux_y_f64(n, rows, cols, v.dx, dt, h, qx, qy, s->dem, s->sqrth, s->rhs[0], s->rhs[1],
+                   s->rhs[2], s->rhs[3], s->rhs[4], s->rhs[5], v.hextra, sp));
+  RC(tr_update_cells_f64(n, rows, cols, dt, h, qx, qy, s->dem, s->nman2, s->rhs[0], s->rhs[1],
+                         s->rhs[2], s->rhs[3], s->rhs[4], s->rhs[5], v.hextra, sp));
+  double step_vol = 0.0;
+  if (v.num_runoffs > 0) {
+    int last = v.roff_nrows - 1;
+    if (s->simtime > s->roff_time[size_t(last)]) s->roff_row = last;
+    else if (s->simtime > s->roff_time[size_t(s->roff_row + 1)]) s->roff_row++;
+    RC(tr_update_runoff_f64(n, rows, cols, dt, s->rid, s->roff_row, v.roff_nrows, s->rrate, h, qx,
+                            qy, v.hextra, sp));
+  }
+  if (v.num_src > 0) {
+    int hi = v.hyg_nrows - 1;
+    bool found = false;
+    for (int i = s->idx_low; i < hi; i++) {
+      if (s->hyg_time[size_t(i + 1)] > s->simtime) { s->idx_low = i; hi = i + 1; found = true; break; }
+    }
+    if (!found) s->idx_low = hi;
+    RC(tr_compute_flow_f64(v.num_src, s->htime, s->hval, v.dx, dt, s->simtime, s->idx_low, hi, h,
+                           s->srcpos, sp));
+  }
+  if (v.open_boundaries) {
+    RC(tr_open_bc_west_east_f64(2 * rows, rows, cols, h, qx, qy, sp));
+    RC(tr_open_bc_north_south_f64(cols, rows, cols, h, qx, qy, 0, 1, sp));
+  }
+  if (v.num_extbc_cells > 0) {
+    RC(tr_compute_extbc_values_f64(v.num_extbc_cells, rows, cols, dt, h, qx, qy, s->dem, s->nman2,
+                                   s->bc_index, s->bc_type, s->bc_start, s->bc_nrows, s->bc_v1,
+                                   s->bc_v2, s->simtime, 0, 1, sp));
+  }
+  RC(tr_wet_dry_f64(n, rows, cols, dt, h, qx, qy, s->dem, s->maxh, v.hextra, 1, sp));
+  (void)step_vol;
+  s->simtime += dt;
+  s->last_dt = dt;
+  s->steps++;
+  if (s->simtime >= v.print_interval * (s->print_id + 1)) s->print_id++;
+  return TR_OK;
+}
+
+// ---- fused FP64 form ------------------------------------------------------
+StepParams<double> fused_params(tr_sim* s, int64_t launch_index) {
+  const tr_case_view& v = s->v;
+  StepParams<double> p{};
+  p.rows = v.rows;
+  p.cols = v.cols;
+  p.strip_rows = s->strip_rows;
+  p.nbx = fused_grid_x(v.cols);
+  p.nby = (v.rows - 2 + s->strip_rows - 1) / s->strip_rows;
+  p.nblocks = unsigned(p.nbx) * unsigned(p.nby);
+  p.dx = v.dx;
+  p.hextra = v.hextra;
+  p.cn_dx = v.courant * v.dx;
+  const int a = int(launch_index & 1);
+  p.h_in = s->h[a];
+  p.qx_in = s->qx[a];
+  p.qy_in = s->qy[a];
+  p.h_out = s->h[a ^ 1];
+  p.qx_out = s->qx[a ^ 1];
+  p.qy_out = s->qy[a ^ 1];
+  p.dem = s->dem;
+  p.nman2 = s->friction ? s->nman2 : nullptr;
+  p.maxh = s->maxh;
+  p.runoff_id = s->rid;
+  p.roff_rate = s->rrate;
+  p.roff_time = s->rtime;
+  p.roff_nrows = v.roff_nrows;
+  p.num_runoffs = v.num_runoffs;
+  p.zone_area = s->zone_area;
+  p.nsrc = v.num_src;
+  p.src_row = s->src_row;
+  p.src_col = s->src_col;
+  p.src_id = s->src_id;
+  p.hyg_time = s->htime;
+  p.hyg_val = s->hval;
+  p.hyg_nrows = v.hyg_nrows;
+  p.src_add = s->src_add;
+  p.open_bc = v.open_boundaries;
+  p.ctrl = s->ctrl;
+  p.cur = a;
+  p.sim_duration = v.sim_duration;
+  p.print_interval = v.print_interval;
+  p.init_dt = v.init_dt;
+  p.t_stop = INFINITY;
+  p.step_stop = INT64_MAX;
+  p.halt_on_print = 0;
+  return p;
+}
+
+int read_ctrl(tr_sim* s, int slot, StepCtrl& out) {
+  TR_HIP(hipMemcpyAsync(s->ctrl_host, s->ctrl + slot, sizeof(StepCtrl), hipMemcpyDeviceToHost,
+                        s->stream));
+  TR_HIP(hipStreamSynchronize(s->stream));
+  out = *s->ctrl_host;
+  return TR_OK;
+}
+
+int write_ctrl(tr_sim* s, int slot, const StepCtrl& c) {
+  *s->ctrl_host = c;
+  TR_HIP(hipMemcpyAsync(s->ctrl + slot, s->ctrl_host, sizeof(StepCtrl), hipMemcpyHostToDevice,
+                        s->stream));
+  TR_HIP(hipStreamSynchronize(s->stream));
+  return TR_OK;
+}
+
+int prime_fused(tr_sim* s) {
+  const tr_case_view& v = s->v;
+  StepCtrl c{};
+  c.simtime = v.sim_start;
+  c.dt = 0.0;
+  c.dtmin_bits = ~0ull;
+  c.arrivals = 0;
+  c.halt = 0;
+  c.step = 0;
+  c.print_id = v.checkpoint_id;
+  if (c.print_id == 0 && c.simtime > 0.0) c.print_id = int(c.simtime / v.print_interval);
+  c.roff_row = v.roff_row0;
+  c.idx_low = 0;
+  c.idx_high = 0;
+  c.vol_in = 0.0;
+  c.step_vol_in = 0.0;
+  RC(write_ctrl(s, 1, c));
+  StepParams<double> p = fused_params(s, 0);
+  p.cur = 1;  // the prime reduces into slot 1 and prepares slot 0 for step 0
+  RC(launch_prime_f64(p, s->stream));
+  TR_HIP(hipStreamSynchronize(s->stream));
+  s->launched = 0;
+  return TR_OK;
+}
+
+// Queue `n` launches without waiting; launches past a halt are no-ops.
+int enqueue_fused(tr_sim* s, int64_t n, double t_stop, int64_t step_stop) {
+  for (int64_t i = 0; i < n; i++) {
+    StepParams<double> p = fused_params(s, s->launched + i);
+    p.t_stop = t_stop;
+    p.step_stop = step_stop;
+    RC(launch_step_f64(p, s->stream));
+  }
+  s->launched += n;
+  return TR_OK;
+}
+
+// After launches: realign the launch counter with the steps really taken, so
+// the state sits in buffer (steps & 1) and its control block in slot (steps & 1).
+int realign_fused(tr_sim* s, StepCtrl& c) {
+  const int slot = int(s->launched & 1);
+  RC(read_ctrl(s, slot, c));
+  if (c.step != s->launched) {
+    const int want = int(c.step & 1);
+    if (want != slot) {
+      RC(write_ctrl(s, want, c));
+      if (s->v.num_src > 0)
+        TR_HIP(hipMemcpyAsync(s->src_add + want * s->v.num_src, s->src_add + slot * s->v.num_src,
+                              sizeof(double) * size_t(s->v.num_src), hipMemcpyDeviceToDevice,
+                              s->stream));
+      TR_HIP(hipStreamSynchronize(s->stream));
+    }
+    s->launched = c.step;
+  }
+  return TR_OK;
+}
+
+int run_fused(tr_sim* s, double t_stop, int64_t max_steps) {
+  StepCtrl c;
+  RC(realign_fused(s, c));
+  const int64_t step_stop = max_steps < 0 ? INT64_MAX : c.step + max_steps;
+  const bool halt = c.finished || !(c.simtime < t_stop) || c.step >= step_stop;
+  c.halt = halt ? 1 : 0;
+  RC(write_ctrl(s, int(s->launched & 1), c));
+  if (halt) return TR_OK;
+  int64_t batch = 32;
+  for (;;) {
+    int64_t n = batch;
+    if (max_steps >= 0) n = std::min<int64_t>(n, step_stop - s->launched);
+    if (n <= 0) n = 1;
+    RC(enqueue_fused(s, n, t_stop, step_stop));
+    RC(realign_fused(s, c));
+    if (c.halt) break;
+    batch = std::min<int64_t>(batch * 2, 1024);
+  }
+  return TR_OK;
+}
+
+int fill_status(tr_sim* s, tr_sim_status* st) {
+  if (!st) return TR_OK;
+  std::memset(st, 0, sizeof(*st));
+  const double* h = s->h[0];
+  if (s->form == TR_FORM_F64_UNFUSED) {
+    st->simtime = s->simtime;
+    st->dt = s->last_dt;
+    st->steps = s->steps;
+    st->print_id = s->print_id;
+    st->finished = !(s->simtime < s->v.sim_duration);
+    st->volume_in = s->vol_in;
+  } else {
+    StepCtrl c;
+    RC(realign_fused(s, c));
+    // the control block of the next step holds the time reached and the
+    // dt that was applied last is the previous block's; report both
+    st->simtime = c.simtime;
+    st->steps = c.step;
+    st->print_id = c.print_id;
+    st->finished = c.finished;
+    st->volume_in = c.vol_in;
+    st->dt = c.dt;
+    h = s->h[c.step & 1];
+  }
+  return volume_f64(s, h, &st->volume);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tr_sim_create(const tr_case_view* view, int32_t form, tr_sim** out) {
+  if (!out) {
+    set_error("tr_sim_create: out is NULL");
+    return TR_ERR_INVALID;
+  }
+  *out = nullptr;
+  RC(check_view(view));
+  if (form != TR_FORM_F64_UNFUSED && form != TR_FORM_F64_FUSED) {
+    set_error("tr_sim_create: form not available in this build");
+    return TR_ERR_INVALID;
+  }
+  if (form != TR_FORM_F64_UNFUSED && view->num_extbc_cells > 0) {
+    set_error("tr_sim_create: external boundary segments run only in TR_FORM_F64_UNFUSED");
+    return TR_ERR_INVALID;
+  }
+  if (form != TR_FORM_F64_UNFUSED && view->num_src > kMaxSrcFused) {
+    set_error("tr_sim_create: too many point sources for the fused step");
+    return TR_ERR_INVALID;
+  }
+  auto* s = new tr_sim();
+  s->form = form;
+  s->v = *view;
+  const tr_case_view& v = *view;
+  const size_t n = size_t(v.rows) * size_t(v.cols);
+  auto fail = [&](int rc) {
+    delete s;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    s->stream = nullptr;
+    set_error("hipStreamCreate failed");
+    return fail(TR_ERR_HIP);
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&s->ctrl_host), sizeof(StepCtrl)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&s->h_pinned), 64) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return fail(TR_ERR_NOMEM);
+  }
+  const int nbuf = form == TR_FORM_F64_UNFUSED ? 1 : 2;
+  int rc = TR_OK;
+  for (int b = 0; b < nbuf && !rc; b++) {
+    rc = s->upload(s->h[b], v.h, n);
+    if (!rc) rc = s->upload(s->qx[b], v.qx, n);
+    if (!rc) rc = s->upload(s->qy[b], v.qy, n);
+  }
+  if (!rc) rc = s->upload(s->dem, v.dem, n);
+  if (!rc) rc = s->upload(s->nman2, v.nman2, n);
+  if (!rc && v.maxh) rc = s->upload(s->maxh, v.maxh, n);  // NULL view.maxh: not tracked
+  for (size_t i = 0; i < n && !s->friction; i++) s->friction = v.nman2[i] > kEps12;
+  // forcing tables
+  if (!rc && v.num_runoffs > 0) {
+    s->roff_time.assign(v.roff_time, v.roff_time + v.roff_nrows);
+    rc = s->upload(s->rrate, v.roff_rate, size_t(v.num_runoffs) * v.roff_nrows);
+    if (!rc) rc = s->upload(s->rtime, v.roff_time, size_t(v.roff_nrows));
+    std::vector<double> area(size_t(v.num_runoffs), 0.0);
+    bool uniform = true;
+    for (int r = 1; r < v.rows - 1 && v.runoff_id; r++)
+      for (int c = 1; c < v.cols - 1; c++) {
+        int z = v.runoff_id[size_t(r) * v.cols + c];
+        area[size_t(z)] += 1.0;
+        uniform = uniform && z == 0;
+      }
+    if (!v.runoff_id) area[0] = double(v.rows - 2) * double(v.cols - 2);
+    for (auto& a : area) a *= v.dx * v.dx;
+    if (!rc) rc = s->upload(s->zone_area, area.data(), area.size());
+    if (!rc && v.runoff_id && (form == TR_FORM_F64_UNFUSED || !uniform))
+      rc = s->upload(s->rid, v.runoff_id, n);
+    if (!rc && form == TR_FORM_F64_UNFUSED && !v.runoff_id) {
+      std::vector<int32_t> zeros(n, 0);
+      rc = s->upload(s->rid, zeros.data(), n);
+    }
+  }
+  if (!rc && v.num_src > 0) {
+    s->hyg_time.assign(v.hyg_time, v.hyg_time + v.hyg_nrows);
+    rc = s->upload(s->htime, v.hyg_time, size_t(v.hyg_nrows));
+    if (!rc) rc = s->upload(s->hval, v.hyg_val, size_t(v.hyg_nrows) * v.num_src);
+    if (!rc) rc = s->upload(s->srcpos, v.src_pos, size_t(v.num_src));
+    // sources sorted by (row, col), stable so one cell's sources keep list order
+    std::vector<int32_t> order(size_t(v.num_src));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return v.src_pos[a] < v.src_pos[b]; });
+    std::vector<int32_t> rr, cc;
+    for (int k : order) {
+      rr.push_back(v.src_pos[k] / v.cols);
+      cc.push_back(v.src_pos[k] % v.cols);
+    }
+    if (!rc) rc = s->upload(s->src_row, rr.data(), rr.size());
+    if (!rc) rc = s->upload(s->src_col, cc.data(), cc.size());
+    if (!rc) rc = s->upload(s->src_id, order.data(), order.size());
+    if (!rc) rc = s->alloc(s->src_add, size_t(2 * v.num_src));
+  }
+  if (!rc && v.num_extbc_cells > 0) {
+    const size_t k = size_t(v.num_extbc_cells);
+    rc = s->upload(s->bc_index, v.bc_index, k);
+    if (!rc) rc = s->upload(s->bc_type, v.bc_type, k);
+    if (!rc) rc = s->upload(s->bc_start, v.bc_start, k);
+    if (!rc) rc = s->upload(s->bc_nrows, v.bc_nrows, k);
+    if (!rc) rc = s->upload(s->bc_v1, v.bc_var1, size_t(std::max(v.bc_nvars, 1)));
+    if (!rc) rc = s->upload(s->bc_v2, v.bc_var2, size_t(std::max(v.bc_nvars, 1)));
+  }
+  if (!rc) rc = s->alloc(s->vpart, kVolBlocks);
+  if (!rc) rc = s->alloc(s->vsum, 1);
+  if (!rc && form == TR_FORM_F64_UNFUSED) {
+    rc = s->alloc(s->sqrth, n);
+    if (!rc) rc = s->alloc(s->dtarr, n);
+    for (int i = 0; i < 6 && !rc; i++) {
+      rc = s->alloc(s->rhs[i], n);
+      if (!rc) {
+        hipError_t e = hipMemsetAsync(s->rhs[i], 0, n * sizeof(double), s->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemsetAsync");
+      }
+    }
+    if (!rc) rc = s->alloc(s->dmin, 1);
+    s->simtime = v.sim_start;
+    s->print_id = v.checkpoint_id;
+    if (s->print_id == 0 && s->simtime > 0.0) s->print_id = int(s->simtime / v.print_interval);
+    s->roff_row = v.roff_row0;
+  }
+  if (!rc && form != TR_FORM_F64_UNFUSED) {
+    rc = s->alloc(s->ctrl, 2);
+    if (!rc) rc = prime_fused(s);
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "upload");
+  }
+  // drop view pointers: the sim owns device copies only
+  s->v.h = s->v.qx = s->v.qy = s->v.dem = s->v.nman2 = s->v.maxh = nullptr;
+  s->v.runoff_id = nullptr; s->v.roff_time = s->v.roff_rate = nullptr;
+  s->v.hyg_time = s->v.hyg_val = nullptr; s->v.src_pos = nullptr;
+  s->v.bc_index = s->v.bc_type = s->v.bc_start = s->v.bc_nrows = nullptr;
+  s->v.bc_var1 = s->v.bc_var2 = nullptr;
+  if (rc) return fail(rc);
+  *out = s;
+  return TR_OK;
+}
+
+int tr_sim_run(tr_sim* s, double t_stop, int64_t max_steps, tr_sim_status* st) {
+  if (!s) {
+    set_error("tr_sim_run: NULL sim");
+    return TR_ERR_INVALID;
+  }
+  const double stop = t_stop > 0 ? std::min(t_stop, s->v.sim_duration) : s->v.sim_duration;
+  if (s->form == TR_FORM_F64_UNFUSED) {
+    int64_t done = 0;
+    while (s->simtime < stop && (max_steps < 0 || done < max_steps)) {
+      RC(step_unfused(s));
+      done++;
+    }
+  } else {
+    RC(run_fused(s, stop, max_steps));
+  }
+  return fill_status(s, st);
+}
+
+int tr_sim_enqueue(tr_sim* s, int64_t nsteps) {
+  if (!s || nsteps < 0) {
+    set_error("tr_sim_enqueue: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (s->form == TR_FORM_F64_UNFUSED) {
+    set_error("tr_sim_enqueue: only fused forms queue steps");
+    return TR_ERR_STATE;
+  }
+  return enqueue_fused(s, nsteps, s->v.sim_duration, INT64_MAX);
+}
+
+int tr_sim_sync(tr_sim* s, tr_sim_status* st) {
+  if (!s) {
+    set_error("tr_sim_sync: NULL sim");
+    return TR_ERR_INVALID;
+  }
+  TR_HIP(hipStreamSynchronize(s->stream));
+  return fill_status(s, st);
+}
+
+int tr_sim_download(tr_sim* s, double* h, double* qx, double* qy, double* maxh) {
+  if (!s) {
+    set_error("tr_sim_download: NULL sim");
+    return TR_ERR_INVALID;
+  }
+  int b = 0;
+  if (s->form != TR_FORM_F64_UNFUSED) {
+    StepCtrl c;
+    RC(realign_fused(s, c));
+    b = int(c.step & 1);
+  }
+  const size_t n = size_t(s->v.rows) * size_t(s->v.cols) * sizeof(double);
+  if (h) TR_HIP(hipMemcpyAsync(h, s->h[b], n, hipMemcpyDeviceToHost, s->stream));
+  if (qx) TR_HIP(hipMemcpyAsync(qx, s->qx[b], n, hipMemcpyDeviceToHost, s->stream));
+  if (qy) TR_HIP(hipMemcpyAsync(qy, s->qy[b], n, hipMemcpyDeviceToHost, s->stream));
+  if (maxh && s->maxh) TR_HIP(hipMemcpyAsync(maxh, s->maxh, n, hipMemcpyDeviceToHost, s->stream));
+  TR_HIP(hipStreamSynchronize(s->stream));
+  return TR_OK;
+}
+
+void* tr_sim_stream(tr_sim* s) { return s ? reinterpret_cast<void*>(s->stream) : nullptr; }
+
+int64_t tr_sim_device_bytes(const tr_sim* s) { return s ? s->bytes : 0; }
+
+void tr_sim_destroy(tr_sim* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// step_ctrl.hpp -- device routines that close a fused step: the wave-level
+// dt min with a single-pass "last wave" detection, and the single-thread
+// bookkeeping that the reference's host loop does between steps
+// (src/triton_simulate.h:54-134, :202-226, :246-285).
+#pragma once
+
+#include "common.hpp"
+#include "step.hpp"
+
+namespace tr {
+
+// Wave64 min of uint64 values via cross-lane shuffles.
+__device__ __forceinline__ unsigned long long wave_min_bits(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Prepare `nx` (the control block of the next step) from `pv` (the step just
+// done, or the initial block when after_step is false).  Single thread.
+template <class Real>
+__device__ void prepare_next(const StepParams<Real>& p, const StepCtrl& pv, StepCtrl& nx,
+                             unsigned long long dtmin_bits, bool after_step, int nxt_slot) {
+  double simtime = after_step ? pv.simtime + pv.dt : pv.simtime;
+  int print_id = pv.print_id;
+  long long step = pv.step + (after_step ? 1 : 0);
+  bool printed = false;
+  if (after_step && simtime >= p.print_interval * (print_id + 1)) {
+    print_id++;
+    printed = true;
+  }
+  // compute_global_dt (src/triton_simulate.h:202-226)
+  double dt = bitsd(dtmin_bits);
+  if (dtmin_bits == ~0ull) dt = kMaxDt;
+  if (dt >= kMaxDt - 1.0) dt = p.init_dt;
+  double next_print = p.print_interval * (print_id + 1);
+  if (simtime + dt > next_print) dt = next_print - simtime;
+
+  // runoff row, advanced by at most one per step (src/triton_simulate.h:246-258)
+  int row = pv.roff_row;
+  double rain_vol = 0.0;
+  if (p.num_runoffs > 0) {
+    int last = p.roff_nrows - 1;
+    if (simtime > p.roff_time[last]) row = last;
+    else if (simtime > p.roff_time[row + 1]) row++;
+    for (int z = 0; z < p.num_runoffs; z++)
+      rain_vol += p.roff_rate[z * p.roff_nrows + row] * dt * p.zone_area[z];
+  }
+  // hydrograph rows and the depth each source adds (src/triton_simulate.h:264-285,
+  // src/kernels.h:60-81)
+  int lo = pv.idx_low, hi = pv.idx_high;
+  double src_vol = 0.0;
+  if (p.nsrc > 0) {
+    hi = p.hyg_nrows - 1;
+    bool found = false;
+    for (int i = lo; i < hi; i++) {
+      if (p.hyg_time[i + 1] > simtime) { lo = i; hi = i + 1; found = true; break; }
+    }
+    if (!found) lo = hi;
+    double* add = p.src_add + nxt_slot * p.nsrc;
+    for (int k = 0; k < p.nsrc; k++) {
+      int s = p.src_id[k];
+      double flo = p.hyg_val[lo * p.nsrc + s], fhi = p.hyg_val[hi * p.nsrc + s];
+      double q;
+      if (lo == hi) q = flo;
+      else q = flo + (((fhi - flo) * (simtime - p.hyg_time[lo])) / (p.hyg_time[hi] - p.hyg_time[lo]));
+      add[k] = (q * dt) / (p.dx * p.dx);
+      src_vol += q * dt;
+    }
+  }
+  bool finished = !(simtime < p.sim_duration);
+  bool halt = finished || !(simtime < p.t_stop) || step >= p.step_stop ||
+              (p.halt_on_print && printed);
+
+  nx.simtime = simtime;
+  nx.dt = dt;
+  nx.dtmin_bits = ~0ull;
+  nx.arrivals = 0u;
+  nx.halt = halt ? 1 : 0;
+  nx.step = step;
+  nx.print_id = print_id;
+  nx.roff_row = row;
+  nx.idx_low = lo;
+  nx.idx_high = hi;
+  nx.finished = finished ? 1 : 0;
+  nx.vol_in = after_step ? pv.vol_in + pv.step_vol_in : pv.vol_in;
+  nx.step_vol_in = rain_vol + src_vol;
+}
+
+// A skipped (halted) step carries its control block and source increments
+// to the other slot unchanged, so launches queued past the halt stay no-ops.
+template <class Real>
+__device__ void carry_halted(const StepParams<Real>& p) {
+  const StepCtrl& pv = p.ctrl[p.cur];
+  StepCtrl& nx = p.ctrl[p.cur ^ 1];
+  nx = pv;
+  nx.dtmin_bits = ~0ull;
+  nx.arrivals = 0u;
+  for (int k = 0; k < p.nsrc; k++)
+    p.src_add[(p.cur ^ 1) * p.nsrc + k] = p.src_add[p.cur * p.nsrc + k];
+}
+
+// Lane 0 of every wave: publish the wave's dt min, count arrivals and let the
+// last wave prepare the next step.
+template <class Real>
+__device__ void close_step(const StepParams<Real>& p, unsigned long long wave_min,
+                           bool after_step, unsigned nwaves) {
+  StepCtrl* cc = p.ctrl + p.cur;
+  if (wave_min != ~0ull) atomicMin(&cc->dtmin_bits, wave_min);
+  __threadfence();
+  unsigned prev = atomicAdd(&cc->arrivals, 1u);
+  if (prev == nwaves - 1) {
+    __threadfence();
+    unsigned long long m = atomicMin(&cc->dtmin_bits, ~0ull);
+    prepare_next(p, *cc, p.ctrl[p.cur ^ 1], m, after_step, p.cur ^ 1);
+  }
+}
+
+}  // namespace tr

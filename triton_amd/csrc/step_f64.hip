@@ -1,0 +1,223 @@
+// step_f64.hip -- fused FP64 timestep, one launch per step.
+//
+// Each 64-lane wave owns a strip of 60 interior columns (lanes 2..61; lanes
+// 0,1,62,63 are the halo) and marches down `strip_rows` rows plus a two-row
+// halo, keeping a sliding window of rows in registers:
+//   load row r -> x faces of row r (neighbour lane via shuffle) -> the y face
+//   between rows r and r-1 -> update row r-1 -> wet/dry and CFL dt of row r-2
+//   -> store row r-2.
+// Every face is computed once per wave and its two sides go to the two cells
+// that share it (the east side reaches the next lane by a cross-lane shift).
+// The two-row/two-lane halo makes the fused pass exact: wet_dry needs the
+// neighbours' post-update depth (src/kernels_wetdry.h:56-75) and the next dt
+// the post-wet_dry discharge (src/kernels.h:176-185).  The expressions are
+// those of the reference in the same order (roe_f64.hpp); with
+// -ffp-contract=off the result equals the unfused sequence bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "roe_f64.hpp"
+#include "step.hpp"
+#include "step_ctrl.hpp"
+
+namespace tr {
+
+constexpr int kLanes = 64;
+constexpr int kOutCols = 60;
+
+// Initial dt: per-cell CFL over every cell (ghosts included), then the last
+// wave prepares the control block of step 0.
+__global__ __launch_bounds__(64) void k_prime_f64(StepParams<double> p) {
+  const int n = p.rows * p.cols;
+  unsigned long long best = ~0ull;
+  for (int id = blockIdx.x * kLanes + threadIdx.x; id < n; id += gridDim.x * kLanes) {
+    double h = p.h_in[id];
+    double sq = fmax(sqrt(h), 0.0);
+    unsigned long long b = dbits(cell_dt_f64(h, p.qx_in[id], p.qy_in[id], sq, p.cn_dx, p.hextra));
+    best = b < best ? b : best;
+  }
+  best = wave_min_bits(best);
+  if (threadIdx.x == 0) close_step(p, best, false, gridDim.x);
+}
+
+template <bool kOpen>
+__global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
+  const int lane = threadIdx.x;
+  const StepCtrl* cc = p.ctrl + p.cur;
+  if (cc->halt) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) carry_halted(p);
+    return;
+  }
+  const double dt = cc->dt;
+  const int roff_row = cc->roff_row;
+  const double* __restrict__ src_add = p.src_add + p.cur * p.nsrc;
+  const double dx = p.dx, hextra = p.hextra;
+  const int R = p.rows, C = p.cols;
+  const int bx = blockIdx.x, by = blockIdx.y;
+
+  const int c = kOutCols * bx - 1 + lane;
+  const bool col_ok = c >= 0 && c < C;
+  const bool col_int = c >= 1 && c <= C - 2;
+  const bool xface_ok = c >= 0 && c <= C - 2 && lane < kLanes - 1;
+  const bool out_lane = lane >= 2 && lane <= kOutCols + 1 && col_int;
+  const bool ghost_w = c == 0 && lane == 1;
+  const bool ghost_e = c == C - 1 && lane >= 3 && lane <= kLanes - 2;
+  const bool col_mine = out_lane || ghost_w || ghost_e;
+
+  const int e_lo = 1 + p.strip_rows * by;
+  const int e_hi = min(e_lo + p.strip_rows - 1, R - 2);
+  const int e_min = by == 0 ? 0 : e_lo;
+  const int e_max = by == p.nby - 1 ? R - 1 : e_hi;
+
+  // previous row of the input state
+  double ph = 0.0, pqx = 0.0, pqy = 0.0, pz = 0.0, psq = 0.0, pn2 = 0.0;
+  double prate = 0.0;
+  // face contributions of the previous row: east side of the x face (rhs*0),
+  // west side (rhs*1), north y face (rhs*0)
+  double px0h = 0.0, px0x = 0.0, px0y = 0.0, px1h = 0.0, px1x = 0.0, px1y = 0.0;
+  double py0h = 0.0, py0x = 0.0, py0y = 0.0;
+  // updated rows r-2 (A) and r-3 (B)
+  double ah = 0.0, aqx = 0.0, aqy = 0.0, az = 0.0;
+  double bh = 0.0, bz = 0.0;
+  unsigned long long best = ~0ull;
+
+  for (int r = e_min - 2; r <= e_max + 2; ++r) {
+    // A: load row r
+    const bool row_ok = r >= 0 && r < R;
+    double sh = 0.0, sqx = 0.0, sqy = 0.0, sz = 0.0, sn2 = 0.0, srate = 0.0;
+    const long long idx = (long long)r * C + c;
+    if (row_ok && col_ok) {
+      sh = p.h_in[idx];
+      sqx = p.qx_in[idx];
+      sqy = p.qy_in[idx];
+      sz = p.dem[idx];
+      if (p.nman2) sn2 = p.nman2[idx];
+      if (p.num_runoffs > 0 && r >= 1 && r <= R - 2 && col_int) {
+        int zone = p.runoff_id ? p.runoff_id[idx] : 0;
+        srate = p.roff_rate[zone * p.roff_nrows + roff_row];
+      }
+    }
+    const double ssq = fmax(sqrt(sh), 0.0);
+
+    // B: x face (c, c+1) of row r
+    double x0h = 0.0, x0x = 0.0, x0y = 0.0, x1h, x1x, x1y;
+    {
+      const double hR = __shfl_down(sh, 1), qxR = __shfl_down(sqx, 1), qyR = __shfl_down(sqy, 1);
+      const double zR = __shfl_down(sz, 1), sqR = __shfl_down(ssq, 1);
+      double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+      if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
+        Face64 f;
+        if (roe_face_f64<1, 0>(sh, hR, sqx, qxR, sqy, qyR, sz, zR, ssq, sqR, dx, dt, hextra, f)) {
+          x0h = dt * f.d1[0] / dx;
+          x0x = dt * f.d1[1] / dx;
+          x0y = dt * f.d1[2] / dx;
+          e0 = dt * f.d2[0] / dx;
+          e1 = dt * f.d2[1] / dx;
+          e2 = dt * f.d2[2] / dx;
+        }
+      }
+      x1h = __shfl_up(e0, 1);
+      x1x = __shfl_up(e1, 1);
+      x1y = __shfl_up(e2, 1);
+    }
+
+    // C: y face between row r (south) and row r-1 (north)
+    double y0h = 0.0, y0x = 0.0, y0y = 0.0, y1h = 0.0, y1x = 0.0, y1y = 0.0;
+    if (row_ok && r >= 1 && col_int) {
+      Face64 f;
+      if (roe_face_f64<0, 1>(sh, ph, sqx, pqx, sqy, pqy, sz, pz, ssq, psq, dx, dt, hextra, f)) {
+        y0h = dt * f.d1[0] / dx;
+        y0x = dt * f.d1[1] / dx;
+        y0y = dt * f.d1[2] / dx;
+        y1h = dt * f.d2[0] / dx;
+        y1x = dt * f.d2[1] / dx;
+        y1y = dt * f.d2[2] / dx;
+      }
+    }
+
+    // D: update row r-1 (update_cells, runoff, point sources)
+    const int ru = r - 1;
+    double nh = ph, nqx = pqx, nqy = pqy;
+    const bool ru_int = ru >= 1 && ru <= R - 2;
+    if (ru_int && col_int) {
+      update_cell_f64(ph, pqx, pqy, pn2, px0h + py0h, px1h + y1h, px0x + py0x, px1x + y1x,
+                      px0y + py0y, px1y + y1y, dt, hextra, nh, nqx, nqy);
+      if (p.num_runoffs > 0) {
+        double hc = nh;
+        hc += prate * dt;
+        if (hc < hextra) {
+          if (hc < 0.0) hc = 0.0;
+          nqx = 0.0;
+          nqy = 0.0;
+        }
+        nh = hc;
+      }
+    }
+    if (ru_int) {
+      for (int k = 0; k < p.nsrc; k++) {
+        if (p.src_row[k] == ru && p.src_col[k] == c) nh = nh + src_add[k];
+      }
+    }
+    if (kOpen) {
+      // transmissive ghosts: west/east columns first, then north/south rows
+      const double rh = __shfl_down(nh, 1), rqx = __shfl_down(nqx, 1), rqy = __shfl_down(nqy, 1);
+      const double lh = __shfl_up(nh, 1), lqx = __shfl_up(nqx, 1), lqy = __shfl_up(nqy, 1);
+      if (ru_int && c == 0) { nh = rh; nqx = rqx; nqy = rqy; }
+      if (ru_int && c == C - 1) { nh = lh; nqx = lqx; nqy = lqy; }
+      if (ru == 1) { ah = nh; aqx = nqx; aqy = nqy; }          // row 0 <- row 1
+      if (ru == R - 1) { nh = ah; nqx = aqx; nqy = aqy; }      // row R-1 <- row R-2
+    }
+
+    // E: wet/dry, CFL dt and store of row r-2 (slot A)
+    const int e = r - 2;
+    if (e >= e_min && e <= e_max && e >= 0) {
+      double hq = ah, qxq = aqx, qyq = aqy;
+      const double hE = __shfl_down(ah, 1), zE = __shfl_down(az, 1);
+      const double hW = __shfl_up(ah, 1), zW = __shfl_up(az, 1);
+      const bool cell_int = e >= 1 && e <= R - 2 && col_int;
+      if (cell_int && hq > hextra) {
+        const double lev = hq + az;
+        if ((lev < zE && hE < kEps12) || (lev < zW && hW < kEps12)) qxq = 0.0;
+        if ((lev < bz && bh < kEps12) || (lev < pz && nh < kEps12)) qyq = 0.0;
+      }
+      if (col_mine) {
+        const long long o = (long long)e * C + c;
+        p.h_out[o] = hq;
+        p.qx_out[o] = qxq;
+        p.qy_out[o] = qyq;
+        if (p.maxh && cell_int && hq > p.maxh[o]) p.maxh[o] = hq;
+        const double sq = fmax(sqrt(hq), 0.0);
+        unsigned long long b = dbits(cell_dt_f64(hq, qxq, qyq, sq, p.cn_dx, hextra));
+        best = b < best ? b : best;
+      }
+    }
+
+    // rotate the window
+    bh = ah; bz = az;
+    ah = nh; aqx = nqx; aqy = nqy; az = pz;
+    ph = sh; pqx = sqx; pqy = sqy; pz = sz; psq = ssq; pn2 = sn2; prate = srate;
+    px0h = x0h; px0x = x0x; px0y = x0y; px1h = x1h; px1x = x1x; px1y = x1y;
+    py0h = y0h; py0x = y0x; py0y = y0y;
+  }
+  best = wave_min_bits(best);
+  if (lane == 0) close_step(p, best, true, p.nblocks);
+}
+
+// Host launchers (called by the simulation driver).
+int launch_prime_f64(const StepParams<double>& p, hipStream_t s) {
+  int64_t n = int64_t(p.rows) * p.cols;
+  unsigned g = unsigned(std::min<int64_t>((n + kLanes - 1) / kLanes, 2048));
+  hipLaunchKernelGGL(k_prime_f64, dim3(g), dim3(kLanes), 0, s, p);
+  return launch_check("prime_f64");
+}
+
+int launch_step_f64(const StepParams<double>& p, hipStream_t s) {
+  dim3 grid(unsigned(p.nbx), unsigned(p.nby));
+  if (p.open_bc) hipLaunchKernelGGL(k_step_f64<true>, grid, dim3(kLanes), 0, s, p);
+  else hipLaunchKernelGGL(k_step_f64<false>, grid, dim3(kLanes), 0, s, p);
+  return launch_check("step_f64");
+}
+
+int fused_grid_x(int cols) { return (cols - 2 + kOutCols - 1) / kOutCols; }
+
+}  // namespace tr
