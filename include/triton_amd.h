@@ -167,8 +167,11 @@ int tr_sim_create(const tr_case_view *view, int32_t form, tr_sim **out);
  * `max_steps` more steps (max_steps < 0: unlimited), exactly as the
  * reference's loop would (dt clamped to print times).  Blocking. */
 int tr_sim_run(tr_sim *s, double t_stop, int64_t max_steps, tr_sim_status *st);
-/* Enqueue exactly `nsteps` steps on the sim's stream without waiting (fused
- * forms; steps past the end of the run are skipped on the device). */
+/* Fused forms: set the limits of the steps queued next (t_stop <= 0: the
+ * case's sim_duration; max_steps < 0: unlimited), then enqueue exactly
+ * `nsteps` launches on the sim's stream without waiting.  Launches past a
+ * limit or past sim_duration are skipped on the device. */
+int tr_sim_arm(tr_sim *s, double t_stop, int64_t max_steps);
 int tr_sim_enqueue(tr_sim *s, int64_t nsteps);
 int tr_sim_sync(tr_sim *s, tr_sim_status *st);
 /* Copy the current state to host (padded, FP64). Any pointer may be NULL. */
@@ -178,6 +181,35 @@ void *tr_sim_stream(tr_sim *s);
 /* Bytes of device memory held. */
 int64_t tr_sim_device_bytes(const tr_sim *s);
 void tr_sim_destroy(tr_sim *s);
+
+/* ---- batched FP32 members: ensembles and serving -------------------------- */
+/* Members of uneven size advance together, one launch per step for all of
+ * them, each with its own adaptive dt, time and forcing (the reference runs
+ * ensemble members as separate MPI jobs, src/Ensify.h:43-72).  Members may be
+ * added and removed between launches; tr_batch_add fails with TR_ERR_NOMEM
+ * when the member would exceed the HBM budget given at creation. */
+typedef struct tr_batch tr_batch;
+int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch **out);
+/* Device bytes a member built from `v` would hold (admission control). */
+int64_t tr_batch_member_bytes(const tr_case_view *v);
+int tr_batch_add(tr_batch *b, const tr_case_view *v, int32_t *id);
+int tr_batch_remove(tr_batch *b, int32_t id);
+/* Limits for the steps queued next, for every member: t_stop (<= 0: each
+ * member's sim_duration) and at most max_steps more steps (< 0: unlimited). */
+int tr_batch_arm(tr_batch *b, double t_stop, int64_t max_steps);
+/* Queue nsteps batched launches without waiting. */
+int tr_batch_enqueue(tr_batch *b, int64_t nsteps);
+/* Wait; members past their limits leave the launch.  n_active may be NULL. */
+int tr_batch_sync(tr_batch *b, int32_t *n_active);
+/* arm + enqueue/sync until no member is active or max_steps were queued. */
+int tr_batch_run(tr_batch *b, double t_stop, int64_t max_steps, int32_t *n_active);
+int tr_batch_status(tr_batch *b, int32_t id, tr_sim_status *st);
+int tr_batch_download(tr_batch *b, int32_t id, double *h, double *qx, double *qy, double *maxh);
+int64_t tr_batch_device_bytes(const tr_batch *b);
+int64_t tr_batch_member_cells(tr_batch *b, int32_t id);
+int32_t tr_batch_work_waves(const tr_batch *b);
+void *tr_batch_stream(tr_batch *b);
+void tr_batch_destroy(tr_batch *b);
 
 #ifdef __cplusplus
 }

@@ -103,12 +103,28 @@ def lib() -> C.CDLL:
         "tr_write_output_binary": (C.c_int, [C.c_char_p, i32, i32, _p_d]),
         "tr_sim_create": (C.c_int, [C.POINTER(CaseView), i32, C.POINTER(vp)]),
         "tr_sim_run": (C.c_int, [vp, f64, i64, C.POINTER(SimStatus)]),
+        "tr_sim_arm": (C.c_int, [vp, f64, i64]),
         "tr_sim_enqueue": (C.c_int, [vp, i64]),
         "tr_sim_sync": (C.c_int, [vp, C.POINTER(SimStatus)]),
         "tr_sim_download": (C.c_int, [vp, _p_d, _p_d, _p_d, _p_d]),
         "tr_sim_stream": (vp, [vp]),
         "tr_sim_device_bytes": (i64, [vp]),
         "tr_sim_destroy": (None, [vp]),
+        "tr_batch_create": (C.c_int, [i64, i32, C.POINTER(vp)]),
+        "tr_batch_member_bytes": (i64, [C.POINTER(CaseView)]),
+        "tr_batch_add": (C.c_int, [vp, C.POINTER(CaseView), C.POINTER(i32)]),
+        "tr_batch_remove": (C.c_int, [vp, i32]),
+        "tr_batch_arm": (C.c_int, [vp, f64, i64]),
+        "tr_batch_enqueue": (C.c_int, [vp, i64]),
+        "tr_batch_sync": (C.c_int, [vp, C.POINTER(i32)]),
+        "tr_batch_run": (C.c_int, [vp, f64, i64, C.POINTER(i32)]),
+        "tr_batch_status": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
+        "tr_batch_download": (C.c_int, [vp, i32, _p_d, _p_d, _p_d, _p_d]),
+        "tr_batch_device_bytes": (i64, [vp]),
+        "tr_batch_member_cells": (i64, [vp, i32]),
+        "tr_batch_work_waves": (i32, [vp]),
+        "tr_batch_stream": (vp, [vp]),
+        "tr_batch_destroy": (None, [vp]),
         "tr_compute_dt_and_sqrt_f64": (C.c_int, [i32, f64, vp, vp, vp, vp, vp, f64, f64, vp]),
         "tr_find_min_dt_f64": (C.c_int, [i32, vp, vp, _p_d, vp]),
         "tr_flux_x_f64": (C.c_int, [i32, i32, i32, f64, f64] + [vp] * 11 + [f64, vp]),
@@ -357,8 +373,17 @@ class Simulation:
         check(lib().tr_sim_run(self._h, float(t_stop), int(max_steps), C.byref(st)))
         return st.as_dict()
 
+    def arm(self, t_stop: float = 0.0, max_steps: int = -1) -> None:
+        """Set the limits for steps queued with enqueue() (fused forms)."""
+        check(lib().tr_sim_arm(self._h, float(t_stop), int(max_steps)))
+
     def enqueue(self, nsteps: int) -> None:
+        """Queue nsteps launches on the sim's stream without waiting."""
         check(lib().tr_sim_enqueue(self._h, int(nsteps)))
+
+    @property
+    def stream(self) -> int:
+        return int(lib().tr_sim_stream(self._h) or 0)
 
     def sync(self) -> dict:
         st = SimStatus()
@@ -374,6 +399,93 @@ class Simulation:
     @property
     def device_bytes(self) -> int:
         return int(lib().tr_sim_device_bytes(self._h))
+
+
+class Batch:
+    """Members of uneven size stepped together in FP32, one launch per step
+    (tr_batch_*).  Members keep their own dt, time and forcing; they may be
+    added and removed between launches, within an HBM budget."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, strip_rows: int = 0):
+        h = C.c_void_p()
+        check(lib().tr_batch_create(int(hbm_budget_bytes), int(strip_rows), C.byref(h)))
+        self._h = h
+        self.members: dict[int, Case] = {}
+        self._views: dict[int, CaseView] = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().tr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def member_bytes(case: Case) -> int:
+        v = case.view()
+        return int(lib().tr_batch_member_bytes(C.byref(v)))
+
+    def add(self, case: Case) -> int:
+        v = case.view()
+        mid = C.c_int32()
+        check(lib().tr_batch_add(self._h, C.byref(v), C.byref(mid)))
+        self.members[mid.value] = case
+        return mid.value
+
+    def remove(self, mid: int) -> None:
+        check(lib().tr_batch_remove(self._h, int(mid)))
+        self.members.pop(mid, None)
+
+    def arm(self, t_stop: float = 0.0, max_steps: int = -1) -> None:
+        check(lib().tr_batch_arm(self._h, float(t_stop), int(max_steps)))
+
+    def enqueue(self, nsteps: int) -> None:
+        check(lib().tr_batch_enqueue(self._h, int(nsteps)))
+
+    def sync(self) -> int:
+        n = C.c_int32()
+        check(lib().tr_batch_sync(self._h, C.byref(n)))
+        return n.value
+
+    def run(self, t_stop: float = 0.0, max_steps: int = -1) -> int:
+        n = C.c_int32()
+        check(lib().tr_batch_run(self._h, float(t_stop), int(max_steps), C.byref(n)))
+        return n.value
+
+    def status(self, mid: int) -> dict:
+        st = SimStatus()
+        check(lib().tr_batch_status(self._h, int(mid), C.byref(st)))
+        return st.as_dict()
+
+    def state(self, mid: int) -> dict[str, np.ndarray]:
+        case = self.members[mid]
+        shape = (case.rows, case.cols)
+        out = {k: np.empty(shape, np.float64) for k in ("h", "qx", "qy")}
+        check(lib().tr_batch_download(self._h, int(mid), *(_ptr(out[k], C.c_double) for k in ("h", "qx", "qy")),
+                                      _ptr(None, C.c_double)))
+        return out
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().tr_batch_device_bytes(self._h))
+
+    @property
+    def work_waves(self) -> int:
+        return int(lib().tr_batch_work_waves(self._h))
+
+    @property
+    def stream(self) -> int:
+        return int(lib().tr_batch_stream(self._h) or 0)
 
 
 class Triton:

@@ -25,7 +25,7 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kernels_ref.hip", "step_f64.hip", "sim.hip"]
+HIP_SOURCES = ["kernels_ref.hip", "step_f64.hip", "step_f32.hip", "batch.hip", "sim.hip"]
 HOST_SOURCES = ["host/case.cpp", "capi.cpp"]
 
 HIP_FLAGS = [

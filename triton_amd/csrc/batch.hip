@@ -1,0 +1,673 @@
+// batch.hip -- members of a batched FP32 run (tr_batch_*).
+//
+// Every member owns its device buffers and a control block pair with a
+// cursor; one launch per step covers all active members through a work
+// table of (member, wave) entries, so members of uneven size advance
+// together while each keeps its own adaptive dt, time and forcing.  Members
+// can be added and removed between launches (the serving loop admits
+// requests this way); admission is checked against an HBM budget.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "common.hpp"
+
+namespace tr {
+namespace {
+
+#define RC(x)            \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
+// Control blocks of all members in one allocation so the host reads them
+// with a single copy at sync points.
+struct alignas(128) CtrlPair {
+  StepCtrl ctrl[2];
+  int cursor;
+  int pad[31];
+};
+
+struct Member {
+  bool used = false;
+  bool active = false;   // in the work table
+  InstF32 inst{};        // host image of the device descriptor
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+  int64_t interior = 0;  // interior cells
+  double sim_duration = 0;
+};
+
+__global__ void k_volume_f32(const float* __restrict__ h, int rows, int cols, double area,
+                             double* __restrict__ part) {
+  __shared__ double sm[256];
+  const int n_int = (rows - 2) * (cols - 2);
+  const int per = (n_int + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(lo + per, n_int);
+  double acc = 0.0;
+  for (int k = lo + int(threadIdx.x); k < hi; k += blockDim.x) {
+    int r = k / (cols - 2) + 1, c = k % (cols - 2) + 1;
+    acc += double(h[(long long)r * cols + c]);
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0] * area;
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, int n, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += part[i];
+  *out = s;
+}
+
+constexpr int kVolBlocks = 256;
+
+}  // namespace
+}  // namespace tr
+
+using namespace tr;
+
+struct tr_batch {
+  hipStream_t stream = nullptr;
+  int64_t budget = 0;          // 0: no limit
+  int strip_rows = 64;
+  std::vector<Member> m;
+  int64_t bytes = 0;           // sum over members + shared
+  InstF32* d_inst = nullptr;
+  CtrlPair* d_ctrl = nullptr;
+  CtrlPair* h_ctrl = nullptr;  // pinned mirror
+  int cap = 0;
+  int2* d_work = nullptr;
+  int2* h_work = nullptr;      // pinned
+  int work_cap = 0;
+  int nwork = 0;
+  bool table_dirty = true;
+  double* d_part = nullptr;
+  double* d_sum = nullptr;
+  double* h_sum = nullptr;
+
+  ~tr_batch() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& mm : m)
+      for (void* p : mm.allocs) hipFree(p);
+    hipFree(d_inst);
+    hipFree(d_ctrl);
+    hipFree(d_work);
+    hipFree(d_part);
+    hipFree(d_sum);
+    if (h_ctrl) hipHostFree(h_ctrl);
+    if (h_work) hipHostFree(h_work);
+    if (h_sum) hipHostFree(h_sum);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int grow(tr_batch* b, int need) {
+  if (need <= b->cap) return TR_OK;
+  int cap = std::max(need, std::max(8, b->cap * 2));
+  InstF32* di = nullptr;
+  CtrlPair* dc = nullptr;
+  CtrlPair* hc = nullptr;
+  TR_HIP(hipStreamSynchronize(b->stream));
+  TR_HIP(hipMalloc(reinterpret_cast<void**>(&di), sizeof(InstF32) * size_t(cap)));
+  TR_HIP(hipMalloc(reinterpret_cast<void**>(&dc), sizeof(CtrlPair) * size_t(cap)));
+  TR_HIP(hipHostMalloc(reinterpret_cast<void**>(&hc), sizeof(CtrlPair) * size_t(cap)));
+  if (b->cap > 0) {
+    TR_HIP(hipMemcpy(dc, b->d_ctrl, sizeof(CtrlPair) * size_t(b->cap), hipMemcpyDeviceToDevice));
+  }
+  hipFree(b->d_inst);
+  hipFree(b->d_ctrl);
+  if (b->h_ctrl) hipHostFree(b->h_ctrl);
+  b->d_inst = di;
+  b->d_ctrl = dc;
+  b->h_ctrl = hc;
+  b->cap = cap;
+  // descriptors point into the control pool: rewrite them all
+  for (int i = 0; i < int(b->m.size()); i++) {
+    if (!b->m[size_t(i)].used) continue;
+    b->m[size_t(i)].inst.ctrl = &b->d_ctrl[i].ctrl[0];
+    b->m[size_t(i)].inst.cursor = &b->d_ctrl[i].cursor;
+    TR_HIP(hipMemcpy(b->d_inst + i, &b->m[size_t(i)].inst, sizeof(InstF32), hipMemcpyHostToDevice));
+  }
+  b->table_dirty = true;
+  return TR_OK;
+}
+
+template <class T>
+int m_alloc(Member& mm, T*& p, size_t n) {
+  p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+  if (e != hipSuccess) {
+    p = nullptr;
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return TR_ERR_NOMEM;
+  }
+  mm.allocs.push_back(p);
+  mm.bytes += int64_t(n * sizeof(T));
+  return TR_OK;
+}
+
+template <class T>
+int m_upload(Member& mm, T*& p, const T* src, size_t n, hipStream_t s) {
+  RC(m_alloc(mm, p, n));
+  if (src && n) TR_HIP(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return TR_OK;
+}
+
+bool uniform_zones(const tr_case_view& v) {
+  if (!v.runoff_id) return true;
+  for (int r = 1; r < v.rows - 1; r++)
+    for (int c = 1; c < v.cols - 1; c++)
+      if (v.runoff_id[size_t(r) * v.cols + c] != 0) return false;
+  return true;
+}
+
+bool has_friction(const tr_case_view& v) {
+  const size_t n = size_t(v.rows) * v.cols;
+  for (size_t i = 0; i < n; i++)
+    if (v.nman2[i] > 1e-6) return true;
+  return false;
+}
+
+int check_member_view(const tr_case_view* v) {
+  if (!v || v->rows < 3 || v->cols < 3 || !v->h || !v->qx || !v->qy || !v->dem || !v->nman2) {
+    set_error("tr_batch_add: view needs rows,cols >= 3 and h, qx, qy, dem, nman2");
+    return TR_ERR_INVALID;
+  }
+  if (int64_t(v->rows) * v->cols > int64_t(INT32_MAX)) {
+    set_error("tr_batch_add: grid exceeds 2^31 cells");
+    return TR_ERR_INVALID;
+  }
+  if (!(v->dx > 0) || !(v->courant > 0) || !(v->print_interval > 0)) {
+    set_error("tr_batch_add: dx, courant and print_interval must be positive");
+    return TR_ERR_INVALID;
+  }
+  if (v->num_extbc_cells > 0) {
+    set_error("tr_batch_add: external boundary segments run only in TR_FORM_F64_UNFUSED");
+    return TR_ERR_INVALID;
+  }
+  if (v->num_runoffs > 0 && (v->roff_nrows < 2 || !v->roff_time || !v->roff_rate)) {
+    set_error("tr_batch_add: runoff needs at least two rows");
+    return TR_ERR_INVALID;
+  }
+  if (v->num_src > 0 && (v->hyg_nrows < 2 || !v->hyg_time || !v->hyg_val || !v->src_pos)) {
+    set_error("tr_batch_add: sources need a hydrograph with at least two rows");
+    return TR_ERR_INVALID;
+  }
+  if (v->num_src > kMaxSrcFused) {
+    set_error("tr_batch_add: too many point sources for the fused step");
+    return TR_ERR_INVALID;
+  }
+  const int64_t n = int64_t(v->rows) * v->cols;
+  for (int i = 0; i < v->num_src; i++) {
+    int r = v->src_pos[i] / v->cols, c = v->src_pos[i] % v->cols;
+    if (v->src_pos[i] < 0 || v->src_pos[i] >= n || r < 1 || r > v->rows - 2 || c < 1 ||
+        c > v->cols - 2) {
+      set_error("tr_batch_add: source cell outside the interior");
+      return TR_ERR_INVALID;
+    }
+  }
+  if (v->num_runoffs > 0 && v->runoff_id) {
+    for (int r = 1; r < v->rows - 1; r++)
+      for (int c = 1; c < v->cols - 1; c++) {
+        int z = v->runoff_id[int64_t(r) * v->cols + c];
+        if (z < 0 || z >= v->num_runoffs) {
+          set_error("tr_batch_add: runoff zone id out of range");
+          return TR_ERR_INVALID;
+        }
+      }
+  }
+  return TR_OK;
+}
+
+int64_t estimate_bytes(const tr_case_view& v, bool fric, bool zones) {
+  const int64_t n = int64_t(v.rows) * v.cols;
+  int64_t b = n * 4 * 8;               // h,qx,qy x2 + dzx + dzy
+  if (fric) b += n * 4;
+  if (v.maxh) b += n * 4;
+  if (zones) b += n * 4;
+  b += int64_t(v.num_runoffs) * (v.roff_nrows + 3) * 8 + int64_t(v.roff_nrows) * 8;
+  b += int64_t(v.num_src) * (v.hyg_nrows + 6) * 8 + int64_t(v.hyg_nrows) * 8;
+  return b + 4096;
+}
+
+int rebuild_work(tr_batch* b) {
+  int total = 0;
+  for (auto& mm : b->m)
+    if (mm.used && mm.active) total += int(mm.inst.nblocks);
+  if (total > b->work_cap) {
+    TR_HIP(hipStreamSynchronize(b->stream));
+    hipFree(b->d_work);
+    if (b->h_work) hipHostFree(b->h_work);
+    b->d_work = nullptr;
+    b->h_work = nullptr;
+    int cap = std::max(total, 1024);
+    TR_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_work), sizeof(int2) * size_t(cap)));
+    TR_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_work), sizeof(int2) * size_t(cap)));
+    b->work_cap = cap;
+  }
+  TR_HIP(hipStreamSynchronize(b->stream));
+  // largest members first: their long strips start earliest
+  std::vector<int> order;
+  for (int i = 0; i < int(b->m.size()); i++)
+    if (b->m[size_t(i)].used && b->m[size_t(i)].active) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return b->m[size_t(x)].inst.nblocks > b->m[size_t(y)].inst.nblocks;
+  });
+  int k = 0;
+  for (int i : order)
+    for (unsigned w = 0; w < b->m[size_t(i)].inst.nblocks; w++) b->h_work[k++] = make_int2(i, int(w));
+  b->nwork = k;
+  if (k) TR_HIP(hipMemcpyAsync(b->d_work, b->h_work, sizeof(int2) * size_t(k), hipMemcpyHostToDevice, b->stream));
+  TR_HIP(hipStreamSynchronize(b->stream));
+  b->table_dirty = false;
+  return TR_OK;
+}
+
+int read_ctrls(tr_batch* b) {
+  if (b->cap == 0) return TR_OK;
+  TR_HIP(hipMemcpyAsync(b->h_ctrl, b->d_ctrl, sizeof(CtrlPair) * size_t(b->cap), hipMemcpyDeviceToHost,
+                        b->stream));
+  TR_HIP(hipStreamSynchronize(b->stream));
+  return TR_OK;
+}
+
+const StepCtrl& cur_ctrl(const tr_batch* b, int i) {
+  const CtrlPair& cp = b->h_ctrl[i];
+  return cp.ctrl[cp.cursor & 1];
+}
+
+int put_ctrl(tr_batch* b, int i) {
+  TR_HIP(hipMemcpyAsync(&b->d_ctrl[i], &b->h_ctrl[i], sizeof(CtrlPair), hipMemcpyHostToDevice, b->stream));
+  return TR_OK;
+}
+
+int put_inst(tr_batch* b, int i) {
+  TR_HIP(hipMemcpyAsync(b->d_inst + i, &b->m[size_t(i)].inst, sizeof(InstF32), hipMemcpyHostToDevice,
+                        b->stream));
+  return TR_OK;
+}
+
+Member* member(tr_batch* b, int id) {
+  if (!b || id < 0 || id >= int(b->m.size()) || !b->m[size_t(id)].used) {
+    set_error("tr_batch: no such member");
+    return nullptr;
+  }
+  return &b->m[size_t(id)];
+}
+
+}  // namespace
+
+extern "C" {
+
+int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch** out) {
+  if (!out || hbm_budget_bytes < 0 || strip_rows < 0) {
+    set_error("tr_batch_create: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  *out = nullptr;
+  auto* b = new tr_batch();
+  b->budget = hbm_budget_bytes;
+  b->strip_rows = strip_rows > 0 ? strip_rows : 64;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    b->stream = nullptr;
+    delete b;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  if (hipMalloc(reinterpret_cast<void**>(&b->d_part), sizeof(double) * kVolBlocks) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&b->d_sum), sizeof(double)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&b->h_sum), sizeof(double)) != hipSuccess) {
+    delete b;
+    set_error("tr_batch_create: allocation failed");
+    return TR_ERR_NOMEM;
+  }
+  int rc = grow(b, 8);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return TR_OK;
+}
+
+int64_t tr_batch_member_bytes(const tr_case_view* v) {
+  if (!v || v->rows < 3 || v->cols < 3 || !v->nman2) return -1;
+  return estimate_bytes(*v, has_friction(*v), !uniform_zones(*v));
+}
+
+int tr_batch_add(tr_batch* b, const tr_case_view* vp, int32_t* id_out) {
+  if (!b || !id_out) {
+    set_error("tr_batch_add: NULL argument");
+    return TR_ERR_INVALID;
+  }
+  RC(check_member_view(vp));
+  const tr_case_view& v = *vp;
+  const bool fric = has_friction(v);
+  const bool zones = v.num_runoffs > 0 && !uniform_zones(v);
+  const int64_t need = estimate_bytes(v, fric, zones);
+  if (b->budget > 0 && b->bytes + need > b->budget) {
+    set_error("tr_batch_add: member needs " + std::to_string(need) + " B, over the HBM budget");
+    return TR_ERR_NOMEM;
+  }
+  int id = -1;
+  for (int i = 0; i < int(b->m.size()); i++)
+    if (!b->m[size_t(i)].used) { id = i; break; }
+  if (id < 0) {
+    id = int(b->m.size());
+    b->m.emplace_back();
+  }
+  RC(grow(b, id + 1));
+  Member& mm = b->m[size_t(id)];
+  mm = Member();
+  hipStream_t s = b->stream;
+  const int rows = v.rows, cols = v.cols;
+  const size_t n = size_t(rows) * cols;
+  InstF32& P = mm.inst;
+  P.rows = rows;
+  P.cols = cols;
+  P.strip_rows = b->strip_rows;
+  P.nbx = fused_grid_x_f32(cols);
+  P.nby = (rows - 2 + P.strip_rows - 1) / P.strip_rows;
+  P.nblocks = unsigned(P.nbx) * unsigned(P.nby);
+  P.dx = v.dx;
+  P.hextra = v.hextra;
+  P.cn_dx = v.courant * v.dx;
+  P.dx_f = float(v.dx);
+  P.hextra_f = float(v.hextra);
+  P.cn_dx_f = float(v.courant * v.dx);
+  // FP32 state; the bed as per-face differences taken in FP64
+  std::vector<float> hf(n), qxf(n), qyf(n), dzx(n, 0.f), dzy(n, 0.f), n2f;
+  for (size_t i = 0; i < n; i++) {
+    hf[i] = float(v.h[i]);
+    qxf[i] = float(v.qx[i]);
+    qyf[i] = float(v.qy[i]);
+  }
+  for (int r = 0; r < rows; r++)
+    for (int c = 0; c < cols; c++) {
+      const size_t i = size_t(r) * cols + c;
+      if (c < cols - 1) dzx[i] = float(v.dem[i + 1] - v.dem[i]);
+      if (r >= 1) dzy[i] = float(v.dem[i - size_t(cols)] - v.dem[i]);
+    }
+  int rc = TR_OK;
+  for (int k = 0; k < 2 && !rc; k++) {
+    rc = m_upload(mm, P.h[k], hf.data(), n, s);
+    if (!rc) rc = m_upload(mm, P.qx[k], qxf.data(), n, s);
+    if (!rc) rc = m_upload(mm, P.qy[k], qyf.data(), n, s);
+  }
+  float* pdx = nullptr;
+  float* pdy = nullptr;
+  if (!rc) rc = m_upload(mm, pdx, dzx.data(), n, s);
+  if (!rc) rc = m_upload(mm, pdy, dzy.data(), n, s);
+  P.dzx = pdx;
+  P.dzy = pdy;
+  P.nman2 = nullptr;
+  if (!rc && fric) {
+    n2f.resize(n);
+    for (size_t i = 0; i < n; i++) n2f[i] = float(v.nman2[i]);
+    float* p = nullptr;
+    rc = m_upload(mm, p, n2f.data(), n, s);
+    P.nman2 = p;
+  }
+  P.maxh = nullptr;
+  if (!rc && v.maxh) {
+    std::vector<float> mf(n);
+    for (size_t i = 0; i < n; i++) mf[i] = float(v.maxh[i]);
+    rc = m_upload(mm, P.maxh, mf.data(), n, s);
+  }
+  // forcing
+  P.num_runoffs = v.num_runoffs;
+  P.roff_nrows = v.roff_nrows;
+  P.runoff_id = nullptr;
+  if (!rc && v.num_runoffs > 0) {
+    double *rate = nullptr, *rt = nullptr, *za = nullptr;
+    rc = m_upload(mm, rate, v.roff_rate, size_t(v.num_runoffs) * v.roff_nrows, s);
+    if (!rc) rc = m_upload(mm, rt, v.roff_time, size_t(v.roff_nrows), s);
+    std::vector<double> area(size_t(v.num_runoffs), 0.0);
+    for (int r = 1; r < rows - 1; r++)
+      for (int c = 1; c < cols - 1; c++) area[size_t(v.runoff_id ? v.runoff_id[size_t(r) * cols + c] : 0)] += 1.0;
+    for (auto& a : area) a *= v.dx * v.dx;
+    if (!rc) rc = m_upload(mm, za, area.data(), area.size(), s);
+    P.roff_rate = rate;
+    P.roff_time = rt;
+    P.zone_area = za;
+    if (!rc) rc = m_alloc(mm, P.zone_add, size_t(2 * v.num_runoffs));
+    if (!rc && zones) {
+      int* rid = nullptr;
+      rc = m_upload(mm, rid, v.runoff_id, n, s);
+      P.runoff_id = rid;
+    }
+  }
+  P.nsrc = v.num_src;
+  P.hyg_nrows = v.hyg_nrows;
+  if (!rc && v.num_src > 0) {
+    std::vector<int> order(size_t(v.num_src));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return v.src_pos[x] < v.src_pos[y]; });
+    std::vector<int> rr, cc;
+    for (int k : order) {
+      rr.push_back(v.src_pos[k] / cols);
+      cc.push_back(v.src_pos[k] % cols);
+    }
+    int *pr = nullptr, *pc = nullptr, *pi = nullptr;
+    double *ht = nullptr, *hv = nullptr;
+    rc = m_upload(mm, pr, rr.data(), rr.size(), s);
+    if (!rc) rc = m_upload(mm, pc, cc.data(), cc.size(), s);
+    if (!rc) rc = m_upload(mm, pi, order.data(), order.size(), s);
+    if (!rc) rc = m_upload(mm, ht, v.hyg_time, size_t(v.hyg_nrows), s);
+    if (!rc) rc = m_upload(mm, hv, v.hyg_val, size_t(v.hyg_nrows) * v.num_src, s);
+    if (!rc) rc = m_alloc(mm, P.src_add, size_t(2 * v.num_src));
+    P.src_row = pr;
+    P.src_col = pc;
+    P.src_id = pi;
+    P.hyg_time = ht;
+    P.hyg_val = hv;
+  }
+  P.open_bc = v.open_boundaries;
+  P.ctrl = &b->d_ctrl[id].ctrl[0];
+  P.cursor = &b->d_ctrl[id].cursor;
+  P.sim_duration = v.sim_duration;
+  P.print_interval = v.print_interval;
+  P.init_dt = v.init_dt;
+  P.t_stop = INFINITY;
+  P.step_stop = INT64_MAX;
+  P.halt_on_print = 0;
+  mm.interior = int64_t(rows - 2) * (cols - 2);
+  mm.sim_duration = v.sim_duration;
+  if (rc) {
+    for (void* p : mm.allocs) hipFree(p);
+    mm = Member();
+    return rc;
+  }
+  mm.used = true;
+  // initial control block in slot 1; the prime prepares slot 0 for step 0
+  CtrlPair& cp = b->h_ctrl[id];
+  std::memset(&cp, 0, sizeof(cp));
+  StepCtrl& c0 = cp.ctrl[1];
+  c0.simtime = v.sim_start;
+  c0.dtmin_bits = ~0ull;
+  c0.print_id = v.checkpoint_id;
+  if (c0.print_id == 0 && c0.simtime > 0.0) c0.print_id = int(c0.simtime / v.print_interval);
+  c0.roff_row = v.roff_row0;
+  cp.ctrl[0].dtmin_bits = ~0ull;
+  cp.cursor = 1;
+  RC(put_ctrl(b, id));
+  RC(put_inst(b, id));
+  RC(launch_prime_f32(b->d_inst, id, P, s));
+  TR_HIP(hipStreamSynchronize(s));
+  b->bytes += mm.bytes;
+  mm.active = true;
+  b->table_dirty = true;
+  *id_out = id;
+  return TR_OK;
+}
+
+int tr_batch_remove(tr_batch* b, int32_t id) {
+  Member* mm = member(b, id);
+  if (!mm) return TR_ERR_INVALID;
+  TR_HIP(hipStreamSynchronize(b->stream));
+  for (void* p : mm->allocs) hipFree(p);
+  b->bytes -= mm->bytes;
+  *mm = Member();
+  b->table_dirty = true;
+  return TR_OK;
+}
+
+// Limits for the launches queued next, for every member: simulated time
+// t_stop (<= 0: each member's sim_duration) and at most max_steps more steps
+// (< 0: unlimited).  Members already at their limit drop out of the table.
+int tr_batch_arm(tr_batch* b, double t_stop, int64_t max_steps) {
+  if (!b) {
+    set_error("tr_batch_arm: NULL batch");
+    return TR_ERR_INVALID;
+  }
+  TR_HIP(hipStreamSynchronize(b->stream));
+  RC(read_ctrls(b));
+  for (int i = 0; i < int(b->m.size()); i++) {
+    Member& mm = b->m[size_t(i)];
+    if (!mm.used) continue;
+    CtrlPair& cp = b->h_ctrl[i];
+    StepCtrl& c = cp.ctrl[cp.cursor & 1];
+    const double stop = t_stop > 0 ? std::min(t_stop, mm.sim_duration) : mm.sim_duration;
+    mm.inst.t_stop = stop;
+    mm.inst.step_stop = max_steps < 0 ? INT64_MAX : c.step + max_steps;
+    const bool halt = c.finished || !(c.simtime < stop) || c.step >= mm.inst.step_stop;
+    c.halt = halt ? 1 : 0;
+    const bool was = mm.active;
+    mm.active = !halt;
+    if (was != mm.active) b->table_dirty = true;
+    RC(put_ctrl(b, i));
+    RC(put_inst(b, i));
+  }
+  TR_HIP(hipStreamSynchronize(b->stream));
+  if (b->table_dirty) RC(rebuild_work(b));
+  return TR_OK;
+}
+
+int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
+  if (!b || nsteps < 0) {
+    set_error("tr_batch_enqueue: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (b->table_dirty) RC(rebuild_work(b));
+  BatchArgs a{b->d_inst, b->d_work, b->nwork};
+  for (int64_t k = 0; k < nsteps; k++) RC(launch_step_f32(a, b->nwork, b->stream));
+  return TR_OK;
+}
+
+// Wait for queued steps; members that reached their limit leave the work
+// table.  n_active gets the number of members still stepping.
+int tr_batch_sync(tr_batch* b, int32_t* n_active) {
+  if (!b) {
+    set_error("tr_batch_sync: NULL batch");
+    return TR_ERR_INVALID;
+  }
+  TR_HIP(hipStreamSynchronize(b->stream));
+  RC(read_ctrls(b));
+  int act = 0;
+  for (int i = 0; i < int(b->m.size()); i++) {
+    Member& mm = b->m[size_t(i)];
+    if (!mm.used) continue;
+    const StepCtrl& c = cur_ctrl(b, i);
+    const bool on = !c.halt;
+    if (on != mm.active) {
+      mm.active = on;
+      b->table_dirty = true;
+    }
+    act += on ? 1 : 0;
+  }
+  if (n_active) *n_active = act;
+  return TR_OK;
+}
+
+int tr_batch_run(tr_batch* b, double t_stop, int64_t max_steps, int32_t* n_active) {
+  RC(tr_batch_arm(b, t_stop, max_steps));
+  int32_t act = 0;
+  RC(tr_batch_sync(b, &act));
+  int64_t batch = 16, done = 0;
+  while (act > 0) {
+    int64_t n = batch;
+    if (max_steps >= 0) n = std::min<int64_t>(n, max_steps - done);
+    if (n <= 0) break;
+    RC(tr_batch_enqueue(b, n));
+    done += n;
+    RC(tr_batch_sync(b, &act));
+    batch = std::min<int64_t>(batch * 2, 512);
+  }
+  if (n_active) *n_active = act;
+  return TR_OK;
+}
+
+int tr_batch_status(tr_batch* b, int32_t id, tr_sim_status* st) {
+  Member* mm = member(b, id);
+  if (!mm || !st) return TR_ERR_INVALID;
+  RC(read_ctrls(b));
+  const StepCtrl& c = cur_ctrl(b, id);
+  std::memset(st, 0, sizeof(*st));
+  st->simtime = c.simtime;
+  st->dt = c.dt;
+  st->steps = c.step;
+  st->print_id = c.print_id;
+  st->finished = c.finished;
+  st->volume_in = c.vol_in;
+  const int cur = b->h_ctrl[id].cursor & 1;
+  hipLaunchKernelGGL(k_volume_f32, dim3(kVolBlocks), dim3(256), 0, b->stream, mm->inst.h[cur],
+                     mm->inst.rows, mm->inst.cols, mm->inst.dx * mm->inst.dx, b->d_part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, b->stream, b->d_part, kVolBlocks, b->d_sum);
+  RC(launch_check("volume_f32"));
+  TR_HIP(hipMemcpyAsync(b->h_sum, b->d_sum, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  TR_HIP(hipStreamSynchronize(b->stream));
+  st->volume = *b->h_sum;
+  return TR_OK;
+}
+
+int tr_batch_download(tr_batch* b, int32_t id, double* h, double* qx, double* qy, double* maxh) {
+  Member* mm = member(b, id);
+  if (!mm) return TR_ERR_INVALID;
+  RC(read_ctrls(b));
+  const int cur = b->h_ctrl[id].cursor & 1;
+  const size_t n = size_t(mm->inst.rows) * mm->inst.cols;
+  std::vector<float> tmp(n);
+  auto get = [&](const float* src, double* dst) -> int {
+    if (!dst || !src) return TR_OK;
+    TR_HIP(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    TR_HIP(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < n; i++) dst[i] = double(tmp[i]);
+    return TR_OK;
+  };
+  RC(get(mm->inst.h[cur], h));
+  RC(get(mm->inst.qx[cur], qx));
+  RC(get(mm->inst.qy[cur], qy));
+  RC(get(mm->inst.maxh, maxh));
+  return TR_OK;
+}
+
+int64_t tr_batch_device_bytes(const tr_batch* b) { return b ? b->bytes : 0; }
+
+int64_t tr_batch_member_cells(tr_batch* b, int32_t id) {
+  Member* mm = member(b, id);
+  return mm ? mm->interior : -1;
+}
+
+int32_t tr_batch_work_waves(const tr_batch* b) { return b ? b->nwork : 0; }
+
+void* tr_batch_stream(tr_batch* b) { return b ? reinterpret_cast<void*>(b->stream) : nullptr; }
+
+void tr_batch_destroy(tr_batch* b) { delete b; }
+
+}  // extern "C"

@@ -61,12 +61,18 @@ struct tr_sim {
   StepCtrl* ctrl = nullptr;
   StepCtrl* ctrl_host = nullptr;  // pinned mirror
   int64_t launched = 0;            // == completed steps between API calls
+  double t_stop = INFINITY;        // limits armed for queued steps
+  int64_t step_stop = INT64_MAX;
   int strip_rows = 64;
   // volume reduction
   double* vpart = nullptr;
   double* vsum = nullptr;
+  // FP32 form: a batch of one member
+  tr_batch* batch = nullptr;
+  int32_t member = -1;
 
   ~tr_sim() {
+    if (batch) tr_batch_destroy(batch);
     if (stream) hipStreamSynchronize(stream);
     for (void* p : allocs) hipFree(p);
     if (ctrl_host) hipHostFree(ctrl_host);
@@ -371,20 +377,29 @@ int realign_fused(tr_sim* s, StepCtrl& c) {
   return TR_OK;
 }
 
-int run_fused(tr_sim* s, double t_stop, int64_t max_steps) {
+// Realign, then set the halt flag of the next step for the given limits.
+int arm_fused(tr_sim* s, double t_stop, int64_t max_steps, bool& halted) {
   StepCtrl c;
   RC(realign_fused(s, c));
-  const int64_t step_stop = max_steps < 0 ? INT64_MAX : c.step + max_steps;
-  const bool halt = c.finished || !(c.simtime < t_stop) || c.step >= step_stop;
-  c.halt = halt ? 1 : 0;
+  s->t_stop = t_stop;
+  s->step_stop = max_steps < 0 ? INT64_MAX : c.step + max_steps;
+  halted = c.finished || !(c.simtime < t_stop) || c.step >= s->step_stop;
+  c.halt = halted ? 1 : 0;
   RC(write_ctrl(s, int(s->launched & 1), c));
-  if (halt) return TR_OK;
+  return TR_OK;
+}
+
+int run_fused(tr_sim* s, double t_stop, int64_t max_steps) {
+  bool halted = false;
+  RC(arm_fused(s, t_stop, max_steps, halted));
+  if (halted) return TR_OK;
+  StepCtrl c;
   int64_t batch = 32;
   for (;;) {
     int64_t n = batch;
-    if (max_steps >= 0) n = std::min<int64_t>(n, step_stop - s->launched);
+    if (max_steps >= 0) n = std::min<int64_t>(n, s->step_stop - s->launched);
     if (n <= 0) n = 1;
-    RC(enqueue_fused(s, n, t_stop, step_stop));
+    RC(enqueue_fused(s, n, s->t_stop, s->step_stop));
     RC(realign_fused(s, c));
     if (c.halt) break;
     batch = std::min<int64_t>(batch * 2, 1024);
@@ -430,8 +445,21 @@ int tr_sim_create(const tr_case_view* view, int32_t form, tr_sim** out) {
   }
   *out = nullptr;
   RC(check_view(view));
+  if (form == TR_FORM_F32_FUSED) {
+    auto* s = new tr_sim();
+    s->form = form;
+    s->v = *view;
+    int rc = tr_batch_create(0, 0, &s->batch);
+    if (!rc) rc = tr_batch_add(s->batch, view, &s->member);
+    if (rc) {
+      delete s;
+      return rc;
+    }
+    *out = s;
+    return TR_OK;
+  }
   if (form != TR_FORM_F64_UNFUSED && form != TR_FORM_F64_FUSED) {
-    set_error("tr_sim_create: form not available in this build");
+    set_error("tr_sim_create: unknown form");
     return TR_ERR_INVALID;
   }
   if (form != TR_FORM_F64_UNFUSED && view->num_extbc_cells > 0) {
@@ -567,6 +595,10 @@ int tr_sim_run(tr_sim* s, double t_stop, int64_t max_steps, tr_sim_status* st) {
     return TR_ERR_INVALID;
   }
   const double stop = t_stop > 0 ? std::min(t_stop, s->v.sim_duration) : s->v.sim_duration;
+  if (s->batch) {
+    RC(tr_batch_run(s->batch, stop, max_steps, nullptr));
+    return st ? tr_batch_status(s->batch, s->member, st) : TR_OK;
+  }
   if (s->form == TR_FORM_F64_UNFUSED) {
     int64_t done = 0;
     while (s->simtime < stop && (max_steps < 0 || done < max_steps)) {
@@ -579,6 +611,21 @@ int tr_sim_run(tr_sim* s, double t_stop, int64_t max_steps, tr_sim_status* st) {
   return fill_status(s, st);
 }
 
+int tr_sim_arm(tr_sim* s, double t_stop, int64_t max_steps) {
+  if (!s) {
+    set_error("tr_sim_arm: NULL sim");
+    return TR_ERR_INVALID;
+  }
+  if (s->form == TR_FORM_F64_UNFUSED) {
+    set_error("tr_sim_arm: only fused forms queue steps");
+    return TR_ERR_STATE;
+  }
+  const double stop = t_stop > 0 ? std::min(t_stop, s->v.sim_duration) : s->v.sim_duration;
+  if (s->batch) return tr_batch_arm(s->batch, stop, max_steps);
+  bool halted = false;
+  return arm_fused(s, stop, max_steps, halted);
+}
+
 int tr_sim_enqueue(tr_sim* s, int64_t nsteps) {
   if (!s || nsteps < 0) {
     set_error("tr_sim_enqueue: bad arguments");
@@ -588,13 +635,18 @@ int tr_sim_enqueue(tr_sim* s, int64_t nsteps) {
     set_error("tr_sim_enqueue: only fused forms queue steps");
     return TR_ERR_STATE;
   }
-  return enqueue_fused(s, nsteps, s->v.sim_duration, INT64_MAX);
+  if (s->batch) return tr_batch_enqueue(s->batch, nsteps);
+  return enqueue_fused(s, nsteps, s->t_stop, s->step_stop);
 }
 
 int tr_sim_sync(tr_sim* s, tr_sim_status* st) {
   if (!s) {
     set_error("tr_sim_sync: NULL sim");
     return TR_ERR_INVALID;
+  }
+  if (s->batch) {
+    RC(tr_batch_sync(s->batch, nullptr));
+    return st ? tr_batch_status(s->batch, s->member, st) : TR_OK;
   }
   TR_HIP(hipStreamSynchronize(s->stream));
   return fill_status(s, st);
@@ -605,6 +657,7 @@ int tr_sim_download(tr_sim* s, double* h, double* qx, double* qy, double* maxh) 
     set_error("tr_sim_download: NULL sim");
     return TR_ERR_INVALID;
   }
+  if (s->batch) return tr_batch_download(s->batch, s->member, h, qx, qy, maxh);
   int b = 0;
   if (s->form != TR_FORM_F64_UNFUSED) {
     StepCtrl c;
@@ -620,9 +673,16 @@ int tr_sim_download(tr_sim* s, double* h, double* qx, double* qy, double* maxh) 
   return TR_OK;
 }
 
-void* tr_sim_stream(tr_sim* s) { return s ? reinterpret_cast<void*>(s->stream) : nullptr; }
+void* tr_sim_stream(tr_sim* s) {
+  if (!s) return nullptr;
+  if (s->batch) return tr_batch_stream(s->batch);
+  return reinterpret_cast<void*>(s->stream);
+}
 
-int64_t tr_sim_device_bytes(const tr_sim* s) { return s ? s->bytes : 0; }
+int64_t tr_sim_device_bytes(const tr_sim* s) {
+  if (!s) return 0;
+  return s->batch ? tr_batch_device_bytes(s->batch) : s->bytes;
+}
 
 void tr_sim_destroy(tr_sim* s) { delete s; }
 

@@ -63,6 +63,7 @@ struct StepParams {
   const double* hyg_val;
   int hyg_nrows;
   double* src_add;          // [2][nsrc] depth added this step, by slot
+  double* zone_add;         // [2][num_runoffs] rain depth this step, by slot (or NULL)
   int open_bc;
   StepCtrl* ctrl;           // [2]
   int cur;                  // slot of this step

@@ -21,8 +21,8 @@ __device__ __forceinline__ unsigned long long wave_min_bits(unsigned long long v
 
 // Prepare `nx` (the control block of the next step) from `pv` (the step just
 // done, or the initial block when after_step is false).  Single thread.
-template <class Real>
-__device__ void prepare_next(const StepParams<Real>& p, const StepCtrl& pv, StepCtrl& nx,
+template <class P>
+__device__ void prepare_next(const P& p, const StepCtrl& pv, StepCtrl& nx,
                              unsigned long long dtmin_bits, bool after_step, int nxt_slot) {
   double simtime = after_step ? pv.simtime + pv.dt : pv.simtime;
   int print_id = pv.print_id;
@@ -46,8 +46,12 @@ __device__ void prepare_next(const StepParams<Real>& p, const StepCtrl& pv, Step
     int last = p.roff_nrows - 1;
     if (simtime > p.roff_time[last]) row = last;
     else if (simtime > p.roff_time[row + 1]) row++;
-    for (int z = 0; z < p.num_runoffs; z++)
-      rain_vol += p.roff_rate[z * p.roff_nrows + row] * dt * p.zone_area[z];
+    for (int z = 0; z < p.num_runoffs; z++) {
+      const double rate = p.roff_rate[z * p.roff_nrows + row];
+      rain_vol += rate * dt * p.zone_area[z];
+      // depth each zone's cells gain this step (src/kernels.h:134-135)
+      if (p.zone_add) p.zone_add[nxt_slot * p.num_runoffs + z] = rate * dt;
+    }
   }
   // hydrograph rows and the depth each source adds (src/triton_simulate.h:264-285,
   // src/kernels.h:60-81)
@@ -92,8 +96,8 @@ __device__ void prepare_next(const StepParams<Real>& p, const StepCtrl& pv, Step
 
 // A skipped (halted) step carries its control block and source increments
 // to the other slot unchanged, so launches queued past the halt stay no-ops.
-template <class Real>
-__device__ void carry_halted(const StepParams<Real>& p) {
+template <class P>
+__device__ void carry_halted(const P& p) {
   const StepCtrl& pv = p.ctrl[p.cur];
   StepCtrl& nx = p.ctrl[p.cur ^ 1];
   nx = pv;
@@ -105,8 +109,8 @@ __device__ void carry_halted(const StepParams<Real>& p) {
 
 // Lane 0 of every wave: publish the wave's dt min, count arrivals and let the
 // last wave prepare the next step.
-template <class Real>
-__device__ void close_step(const StepParams<Real>& p, unsigned long long wave_min,
+template <class P>
+__device__ void close_step(const P& p, unsigned long long wave_min,
                            bool after_step, unsigned nwaves) {
   StepCtrl* cc = p.ctrl + p.cur;
   if (wave_min != ~0ull) atomicMin(&cc->dtmin_bits, wave_min);
@@ -116,6 +120,26 @@ __device__ void close_step(const StepParams<Real>& p, unsigned long long wave_mi
     __threadfence();
     unsigned long long m = atomicMin(&cc->dtmin_bits, ~0ull);
     prepare_next(p, *cc, p.ctrl[p.cur ^ 1], m, after_step, p.cur ^ 1);
+  }
+}
+
+// Batched instances keep a device-side cursor: every wave of a step reads
+// `*cursor` when it starts, so the last wave to arrive (all others have read
+// it) can prepare the other slot and flip the cursor.  Halted instances are
+// simply skipped, with nothing to carry.
+template <class P>
+__device__ void close_step_cursor(const P& p, int cur, unsigned long long wave_min,
+                                  bool after_step) {
+  StepCtrl* cc = p.ctrl + cur;
+  if (wave_min != ~0ull) atomicMin(&cc->dtmin_bits, wave_min);
+  __threadfence();
+  unsigned prev = atomicAdd(&cc->arrivals, 1u);
+  if (prev == p.nblocks - 1) {
+    __threadfence();
+    unsigned long long m = atomicMin(&cc->dtmin_bits, ~0ull);
+    prepare_next(p, *cc, p.ctrl[cur ^ 1], m, after_step, cur ^ 1);
+    __threadfence();
+    *p.cursor = cur ^ 1;
   }
 }
 
