@@ -161,8 +161,18 @@ typedef struct tr_sim_status {
   double volume_in;     /* FP64 integrated rain + inflow volume (m^3) */
 } tr_sim_status;
 
+/* Tuning of one simulation / batch member (zero fields take the defaults). */
+typedef struct tr_tuning {
+  int32_t strip_rows;  /* rows per wave of the fused step (0: by grid size) */
+  double dry_eps;      /* FP32 form: dry / flush threshold (0: 1e-10; the
+                          reference's FP32 build uses 1e-6, src/constants.h:43,
+                          which loses mass on small-scale flows) */
+} tr_tuning;
+
 /* Allocate device state for `view` in the given form and upload it. */
 int tr_sim_create(const tr_case_view *view, int32_t form, tr_sim **out);
+int tr_sim_create_ex(const tr_case_view *view, int32_t form, const tr_tuning *tune,
+                     tr_sim **out);
 /* Advance until simtime >= t_stop (t_stop <= 0: the case's sim_duration) or
  * `max_steps` more steps (max_steps < 0: unlimited), exactly as the
  * reference's loop would (dt clamped to print times).  Blocking. */
@@ -182,6 +192,14 @@ void *tr_sim_stream(tr_sim *s);
 int64_t tr_sim_device_bytes(const tr_sim *s);
 void tr_sim_destroy(tr_sim *s);
 
+/* ---- synthetic inputs ------------------------------------------------------ */
+/* No counterpart in the reference: drains a synthetic DEM (nr x nc, row-major)
+ * by priority-flood depression filling with an `eps` gradient and returns the
+ * filled bed and the D8 contributing area in cells; used to carve channel
+ * networks into generated benchmark terrain. */
+int tr_dem_flow_route(const double *dem, int32_t nr, int32_t nc, double eps, double *filled,
+                      double *acc);
+
 /* ---- batched FP32 members: ensembles and serving -------------------------- */
 /* Members of uneven size advance together, one launch per step for all of
  * them, each with its own adaptive dt, time and forcing (the reference runs
@@ -193,6 +211,10 @@ int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch **out
 /* Device bytes a member built from `v` would hold (admission control). */
 int64_t tr_batch_member_bytes(const tr_case_view *v);
 int tr_batch_add(tr_batch *b, const tr_case_view *v, int32_t *id);
+int tr_batch_add_ex(tr_batch *b, const tr_case_view *v, const tr_tuning *tune, int32_t *id);
+/* Default rows per wave for a padded rows x cols grid (enough waves to fill
+ * the device; long strips otherwise). */
+int32_t tr_default_strip_rows(int32_t rows, int32_t cols);
 int tr_batch_remove(tr_batch *b, int32_t id);
 /* Limits for the steps queued next, for every member: t_stop (<= 0: each
  * member's sim_duration) and at most max_steps more steps (< 0: unlimited). */

@@ -67,6 +67,12 @@ class CaseView(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """tr_tuning: rows per wave of the fused step and the FP32 dry threshold."""
+
+    _fields_ = [("strip_rows", C.c_int32), ("dry_eps", C.c_double)]
+
+
 class SimStatus(C.Structure):
     _fields_ = [
         ("simtime", C.c_double), ("dt", C.c_double), ("steps", C.c_int64),
@@ -102,6 +108,10 @@ def lib() -> C.CDLL:
         "tr_write_output_ascii": (C.c_int, [C.c_char_p, i32, i32, _p_d]),
         "tr_write_output_binary": (C.c_int, [C.c_char_p, i32, i32, _p_d]),
         "tr_sim_create": (C.c_int, [C.POINTER(CaseView), i32, C.POINTER(vp)]),
+        "tr_sim_create_ex": (C.c_int, [C.POINTER(CaseView), i32, C.POINTER(Tuning), C.POINTER(vp)]),
+        "tr_batch_add_ex": (C.c_int, [vp, C.POINTER(CaseView), C.POINTER(Tuning), C.POINTER(i32)]),
+        "tr_default_strip_rows": (i32, [i32, i32]),
+        "tr_dem_flow_route": (C.c_int, [_p_d, i32, i32, f64, _p_d, _p_d]),
         "tr_sim_run": (C.c_int, [vp, f64, i64, C.POINTER(SimStatus)]),
         "tr_sim_arm": (C.c_int, [vp, f64, i64]),
         "tr_sim_enqueue": (C.c_int, [vp, i64]),
@@ -343,12 +353,14 @@ class Case:
 class Simulation:
     """A device-resident simulation of one case (tr_sim_*)."""
 
-    def __init__(self, case: Case, form: int = FORM_F64_FUSED):
+    def __init__(self, case: Case, form: int = FORM_F64_FUSED, strip_rows: int = 0,
+                 dry_eps: float = 0.0):
         self.case = case
         self.form = form
         self._view = case.view()
         h = C.c_void_p()
-        check(lib().tr_sim_create(C.byref(self._view), int(form), C.byref(h)))
+        tune = Tuning(int(strip_rows), float(dry_eps))
+        check(lib().tr_sim_create_ex(C.byref(self._view), int(form), C.byref(tune), C.byref(h)))
         self._h = h
 
     def close(self) -> None:
@@ -435,10 +447,11 @@ class Batch:
         v = case.view()
         return int(lib().tr_batch_member_bytes(C.byref(v)))
 
-    def add(self, case: Case) -> int:
+    def add(self, case: Case, strip_rows: int = 0, dry_eps: float = 0.0) -> int:
         v = case.view()
         mid = C.c_int32()
-        check(lib().tr_batch_add(self._h, C.byref(v), C.byref(mid)))
+        tune = Tuning(int(strip_rows), float(dry_eps))
+        check(lib().tr_batch_add_ex(self._h, C.byref(v), C.byref(tune), C.byref(mid)))
         self.members[mid.value] = case
         return mid.value
 

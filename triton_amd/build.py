@@ -26,14 +26,14 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["kernels_ref.hip", "step_f64.hip", "step_f32.hip", "batch.hip", "sim.hip"]
-HOST_SOURCES = ["host/case.cpp", "capi.cpp"]
+HOST_SOURCES = ["host/case.cpp", "host/terrain.cpp", "capi.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", f"-I{ROOT / 'include'}",
 ]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}"]
+HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -74,6 +74,12 @@ __global__ void k_sum_parts(const double* __restrict__ part, int n, double* out)
 
 constexpr int kVolBlocks = 256;
 
+// FP32 dry / flush threshold.  1e-6 (the reference's FP32 EPS12) flushes
+// fluxes of small-scale flows and loses mass (2e-4 relative on the 0.1 m deep
+// paraboloid fixture); 1e-12 lets FP32 round-off through as spurious flow;
+// 1e-10 kept the relative mass error at 5e-8 there.
+constexpr float kDefaultDryEpsF32 = 1e-10f;
+
 }  // namespace
 }  // namespace tr
 
@@ -316,14 +322,14 @@ Member* member(tr_batch* b, int id) {
 extern "C" {
 
 int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch** out) {
-  if (!out || hbm_budget_bytes < 0 || strip_rows < 0) {
+  if (!out || hbm_budget_bytes < 0 || strip_rows < 0 || strip_rows > 4096) {
     set_error("tr_batch_create: bad arguments");
     return TR_ERR_INVALID;
   }
   *out = nullptr;
   auto* b = new tr_batch();
   b->budget = hbm_budget_bytes;
-  b->strip_rows = strip_rows > 0 ? strip_rows : 64;
+  b->strip_rows = strip_rows;  // 0: chosen per member by grid size
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     b->stream = nullptr;
@@ -351,7 +357,15 @@ int64_t tr_batch_member_bytes(const tr_case_view* v) {
   return estimate_bytes(*v, has_friction(*v), !uniform_zones(*v));
 }
 
+int32_t tr_default_strip_rows(int32_t rows, int32_t cols) {
+  return rows >= 3 && cols >= 3 ? default_strip_rows(rows, cols) : 0;
+}
+
 int tr_batch_add(tr_batch* b, const tr_case_view* vp, int32_t* id_out) {
+  return tr_batch_add_ex(b, vp, nullptr, id_out);
+}
+
+int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, int32_t* id_out) {
   if (!b || !id_out) {
     set_error("tr_batch_add: NULL argument");
     return TR_ERR_INVALID;
@@ -381,7 +395,9 @@ int tr_batch_add(tr_batch* b, const tr_case_view* vp, int32_t* id_out) {
   InstF32& P = mm.inst;
   P.rows = rows;
   P.cols = cols;
-  P.strip_rows = b->strip_rows;
+  P.strip_rows = (tune && tune->strip_rows > 0) ? tune->strip_rows
+                 : (b->strip_rows > 0 ? b->strip_rows : default_strip_rows(rows, cols));
+  P.eps_f = (tune && tune->dry_eps > 0) ? float(tune->dry_eps) : kDefaultDryEpsF32;
   P.nbx = fused_grid_x_f32(cols);
   P.nby = (rows - 2 + P.strip_rows - 1) / P.strip_rows;
   P.nblocks = unsigned(P.nbx) * unsigned(P.nby);

@@ -19,6 +19,7 @@ struct InstF32 {
   unsigned nblocks;
   double dx, hextra, cn_dx;  // FP64 copies for the step bookkeeping
   float dx_f, hextra_f, cn_dx_f;
+  float eps_f;               // dry / flush threshold
   float* h[2];               // ping-pong state, FP32
   float* qx[2];
   float* qy[2];
@@ -57,5 +58,6 @@ struct BatchArgs {
 int launch_step_f32(const BatchArgs& a, int nwork, hipStream_t s);
 int launch_prime_f32(const InstF32* d_inst, int member, const InstF32& host_inst, hipStream_t s);
 int fused_grid_x_f32(int cols);
+int default_strip_rows(int rows, int cols);
 
 }  // namespace tr

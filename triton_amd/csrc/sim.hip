@@ -439,6 +439,11 @@ int fill_status(tr_sim* s, tr_sim_status* st) {
 extern "C" {
 
 int tr_sim_create(const tr_case_view* view, int32_t form, tr_sim** out) {
+  return tr_sim_create_ex(view, form, nullptr, out);
+}
+
+int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tune,
+                     tr_sim** out) {
   if (!out) {
     set_error("tr_sim_create: out is NULL");
     return TR_ERR_INVALID;
@@ -450,7 +455,7 @@ int tr_sim_create(const tr_case_view* view, int32_t form, tr_sim** out) {
     s->form = form;
     s->v = *view;
     int rc = tr_batch_create(0, 0, &s->batch);
-    if (!rc) rc = tr_batch_add(s->batch, view, &s->member);
+    if (!rc) rc = tr_batch_add_ex(s->batch, view, tune, &s->member);
     if (rc) {
       delete s;
       return rc;
@@ -470,10 +475,16 @@ int tr_sim_create(const tr_case_view* view, int32_t form, tr_sim** out) {
     set_error("tr_sim_create: too many point sources for the fused step");
     return TR_ERR_INVALID;
   }
+  if (tune && (tune->strip_rows < 0 || tune->strip_rows > 4096)) {
+    set_error("tr_sim_create_ex: strip_rows must be in [0, 4096]");
+    return TR_ERR_INVALID;
+  }
   auto* s = new tr_sim();
   s->form = form;
   s->v = *view;
   const tr_case_view& v = *view;
+  s->strip_rows = (tune && tune->strip_rows > 0) ? tune->strip_rows
+                                                 : tr_default_strip_rows(v.rows, v.cols);
   const size_t n = size_t(v.rows) * size_t(v.cols);
   auto fail = [&](int rc) {
     delete s;

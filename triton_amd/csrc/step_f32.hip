@@ -16,8 +16,8 @@
 // written in normal/tangential components so x and y faces share one routine.
 // Semantics follow the reference (src/kernels_flux.h, src/kernels_state.h,
 // src/kernels_wetdry.h, src/kernels.h:102-187) with these FP32 choices:
-//   * the dry / flush threshold is 1e-6 (the reference's FP32 EPS12,
-//     src/constants.h:43) instead of 1e-12;
+//   * the dry / flush threshold is a member setting (default 1e-10) instead
+//     of 1e-12 (the reference's FP32 build uses 1e-6, src/constants.h:43);
 //   * bed tests z1+h1<z2 become h1<dz on the stored face difference;
 //   * the semi-implicit friction uses q = 2m/(1+sqrt(1+4tt)), the same value
 //     as -0.5(m - m sqrt(1+4tt))/tt without its cancellation for small tt.
@@ -36,7 +36,6 @@ constexpr int kLanes = 64;
 constexpr int kOutCols = 60;
 constexpr float kGf = 9.81f;
 constexpr float kSqrtGf = 3.132091953f;
-constexpr float kEpsF = 1e-6f;
 constexpr float kMaxDtF = 1e10f;
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -50,8 +49,8 @@ struct Side {
 // the cell behind the face (l*) and in front of it (r*): mass, normal and
 // tangential momentum, before the dt/dx scaling.
 __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, float hdxdt,
-                                       float& lm, float& ln, float& lt, float& rm, float& rn,
-                                       float& rt) {
+                                       float kEpsF, float& lm, float& ln, float& lt, float& rm,
+                                       float& rn, float& rt) {
   lm = ln = lt = rm = rn = rt = 0.f;
   if (!(L.h > 0.f || R.h > 0.f)) return;
   const float c = sqrtf_hw(kGf * (0.5f * (L.h + R.h)));
@@ -168,6 +167,7 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
   const float hdxdt = float(0.5 * P.dx / dt64);
   const float hextra = P.hextra_f;
   const float cn_dx = P.cn_dx_f;
+  const float kEpsF = P.eps_f;
   const double* __restrict__ src_add = P.src_add + cur * P.nsrc;
   // rain depth of this step (FP64 product rate*dt, rounded once); zone 0
   // when the member has one uniform zone
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
       if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
         const Side Lt{s.h, s.qx, s.qy, u, v, sq};
         float lm, ln, lt;
-        face32(Lt, Rt, s.dzx, hdxdt, lm, ln, lt, em, en, et);
+        face32(Lt, Rt, s.dzx, hdxdt, kEpsF, lm, ln, lt, em, en, et);
         x0h = lm * dtdx;
         x0x = ln * dtdx;
         x0y = lt * dtdx;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
       const Side S{s.h, s.qy, s.qx, v, u, sq};
       const Side N{ph, pqy, pqx, pv, pu, psq};
       float lm, ln, lt, rm, rn, rt;
-      face32(S, N, s.dzy, hdxdt, lm, ln, lt, rm, rn, rt);
+      face32(S, N, s.dzy, hdxdt, kEpsF, lm, ln, lt, rm, rn, rt);
       y0h = lm * dtdx; y0y = ln * dtdx; y0x = lt * dtdx;
       y1h = rm * dtdx; y1y = rn * dtdx; y1x = rt * dtdx;
     }
@@ -377,6 +377,16 @@ __global__ __launch_bounds__(64) void k_prime_f32(const InstF32* inst, int membe
 }  // namespace
 
 int fused_grid_x_f32(int cols) { return (cols - 2 + kOutCols - 1) / kOutCols; }
+
+// Rows per wave: long strips amortise the two-row halo, but a grid must also
+// give the device enough waves (about 8 per CU) to hide memory latency.
+int default_strip_rows(int rows, int cols) {
+  const int nbx = fused_grid_x_f32(cols);
+  const int target_waves = 2048;
+  int want_nby = (target_waves + nbx - 1) / nbx;
+  int strip = (rows - 2 + want_nby - 1) / want_nby;
+  return strip < 16 ? 16 : (strip > 64 ? 64 : strip);
+}
 
 int launch_step_f32(const BatchArgs& a, int nwork, hipStream_t s) {
   if (nwork <= 0) return TR_OK;

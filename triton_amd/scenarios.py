@@ -66,3 +66,118 @@ def thacker(n: int = 1024, L: float = 8000.0, h0: float = 10.0, a: float = 3000.
     _, T = thacker_params(h0, a, eta)
     return Case.from_arrays(dem=z, h=h, qx=qx, qy=qy, dx=dx, courant=courant, hextra=1e-4,
                             sim_duration=periods * T, print_interval=T / 4.0)
+
+
+# ---- watershed (fBm terrain, channels, land cover, rain, inflows) ----------
+
+def fbm(n: int, hurst: float, seed: int) -> np.ndarray:
+    """Fractional Brownian surface by spectral synthesis: random phases with
+    amplitude |k|^-(H+1) (power ~ k^-(2H+2)), scaled to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(n)[None, :]
+    ky = np.fft.fftfreq(n)[:, None]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** (-(hurst + 1.0))
+    amp[0, 0] = 0.0
+    phase = rng.uniform(0.0, 2.0 * np.pi, size=(n, n))
+    f = np.fft.ifft2(amp * np.exp(1j * phase)).real
+    f -= f.min()
+    f /= max(f.max(), 1e-300)
+    return f
+
+
+def drainage(dem: np.ndarray, eps: float = 1e-5):
+    """Depression-filled bed and D8 contributing area (cells)."""
+    from . import _ptr, check, lib
+    import ctypes as C
+    nr, nc = dem.shape
+    dem = np.ascontiguousarray(dem, np.float64)
+    filled = np.empty_like(dem)
+    acc = np.empty_like(dem)
+    check(lib().tr_dem_flow_route(_ptr(dem, C.c_double), nr, nc, float(eps), _ptr(filled, C.c_double),
+                                  _ptr(acc, C.c_double)))
+    return filled, acc
+
+
+MANNING_CLASSES = (0.02, 0.04, 0.06, 0.08, 0.10)
+
+
+def watershed_inputs(n: int, seed: int, dx: float = 10.0, relief: float = 300.0,
+                     hurst: float = 0.8, channel_area: int = 1000):
+    """Terrain and land cover of the synthetic watershed.
+
+    Bed: fBm (Hurst 0.8) rescaled to 0..relief m, depressions filled so every
+    cell drains to the edge, then channels carved where the contributing area
+    reaches `channel_area` cells, depth 0.5 + 0.5 log10(A/A0) m capped at 5 m.
+    Manning n: a second fBm split at its quantiles into the 5 classes
+    0.02..0.10."""
+    z = fbm(n, hurst, seed) * relief
+    filled, acc = drainage(z)
+    depth = np.where(acc >= channel_area,
+                     np.minimum(0.5 + 0.5 * np.log10(np.maximum(acc, 1.0) / channel_area), 5.0), 0.0)
+    bed = filled - depth
+    land = fbm(n, hurst, seed + 30)
+    edges = np.quantile(land, [0.2, 0.4, 0.6, 0.8])
+    cls = np.searchsorted(edges, land)
+    nman = np.asarray(MANNING_CLASSES)[cls]
+    return bed, nman, acc
+
+
+def triangular(t_h, t0, tp, t1, peak):
+    """Triangle 0 at t0, `peak` at tp, 0 at t1 (times in hours)."""
+    t = np.asarray(t_h, np.float64)
+    up = np.clip((t - t0) / (tp - t0), 0.0, 1.0)
+    dn = np.clip((t1 - t) / (t1 - tp), 0.0, 1.0)
+    return peak * np.where(t <= tp, up, dn) * ((t >= t0) & (t <= t1))
+
+
+def watershed(n: int = 4096, seed: int = 3, dx: float = 10.0, hours: float = 6.0,
+              rain_peak_mmh: float = 50.0, inflow_peaks=(100.0, 200.0, 350.0, 500.0),
+              manning_scale: float = 1.0, courant: float = 0.5, hextra: float = 1e-3,
+              inflow_seed: int = 333, track_maxh: bool = False) -> Case:
+    """Synthetic watershed: n x n cells at dx, dry start, open boundaries.
+
+    Rain: one uniform zone, rows every 60 s, triangular 0 -> peak at 1 h -> 0
+    at 2 h, then dry (applied as the reference's left-continuous staircase).
+    Inflows: triangular hydrographs 0 at 0.5 h -> peak at 2 h -> 0 at 5 h,
+    rows every 10 min, at distinct high-accumulation channel cells."""
+    bed, nman, acc = watershed_inputs(n, seed, dx)
+    nman = nman * manning_scale
+    t_roff_h = np.arange(0.0, hours + 1e-9, 1.0 / 60.0)
+    rate_mmh = triangular(t_roff_h, 0.0, 1.0, 2.0, rain_peak_mmh)
+    runoff = dict(time_s=t_roff_h * 3600.0, rate_ms=[rate_mmh / 3600.0 * 0.001], zone=None)
+    sources = None
+    if inflow_peaks:
+        rng = np.random.default_rng(inflow_seed)
+        interior = np.zeros_like(acc, bool)
+        m = max(2, n // 16)
+        interior[m:-m, m:-m] = True
+        cand = np.flatnonzero((acc >= np.quantile(acc[interior], 0.999)) & interior)
+        pick = rng.choice(cand, size=len(inflow_peaks), replace=False)
+        t_hyg_h = np.arange(0.0, hours + 1e-9, 1.0 / 6.0)
+        q = np.stack([triangular(t_hyg_h, 0.5, 2.0, 5.0, p) for p in inflow_peaks], axis=1)
+        sources = dict(rows=list(pick // n), cols=list(pick % n), time_s=t_hyg_h * 3600.0, q=q)
+    return Case.from_arrays(dem=bed, h=np.zeros((n, n)), nman=nman, dx=dx, courant=courant,
+                            hextra=hextra, sim_duration=hours * 3600.0, print_interval=hours * 3600.0,
+                            open_boundaries=1, runoff=runoff, sources=sources, track_maxh=track_maxh)
+
+
+def ensemble_sizes(count: int = 64, seed: int = 4, lo_exp: float = 8.0, hi_exp: float = 11.0):
+    """Square sides log-uniform between 2^lo_exp and 2^hi_exp."""
+    rng = np.random.default_rng(seed)
+    return [int(round(2.0 ** rng.uniform(lo_exp, hi_exp))) for _ in range(count)]
+
+
+def ensemble(count: int = 64, hours: float = 2.0, seed: int = 4, sizes=None):
+    """Ensemble members: seeds 4000+i, log-uniform sides 256..2048, Manning
+    scale U(0.7, 1.3), rain peak U(20, 80) mm/h, no point inflows."""
+    rng = np.random.default_rng(seed + 1)
+    sizes = sizes or ensemble_sizes(count, seed)
+    out = []
+    for i, side in enumerate(sizes):
+        ms = float(rng.uniform(0.7, 1.3))
+        peak = float(rng.uniform(20.0, 80.0))
+        out.append(watershed(n=side, seed=4000 + i, hours=hours, rain_peak_mmh=peak, inflow_peaks=(),
+                             manning_scale=ms))
+    return out
