@@ -80,6 +80,10 @@ constexpr int kVolBlocks = 256;
 // 1e-10 kept the relative mass error at 5e-8 there.
 constexpr float kDefaultDryEpsF32 = 1e-10f;
 
+// Rows per wave for batch members (the longest strip measured fastest on a
+// full device).
+constexpr int kBatchStripRows = 96;
+
 }  // namespace
 }  // namespace tr
 
@@ -329,7 +333,7 @@ int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch** out
   *out = nullptr;
   auto* b = new tr_batch();
   b->budget = hbm_budget_bytes;
-  b->strip_rows = strip_rows;  // 0: chosen per member by grid size
+  b->strip_rows = strip_rows;  // 0: kBatchStripRows unless a member says otherwise
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     b->stream = nullptr;
@@ -395,8 +399,9 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   InstF32& P = mm.inst;
   P.rows = rows;
   P.cols = cols;
+  // members of a batch share the device, so each takes long strips
   P.strip_rows = (tune && tune->strip_rows > 0) ? tune->strip_rows
-                 : (b->strip_rows > 0 ? b->strip_rows : default_strip_rows(rows, cols));
+                 : (b->strip_rows > 0 ? b->strip_rows : kBatchStripRows);
   P.eps_f = (tune && tune->dry_eps > 0) ? float(tune->dry_eps) : kDefaultDryEpsF32;
   P.nbx = fused_grid_x_f32(cols);
   P.nby = (rows - 2 + P.strip_rows - 1) / P.strip_rows;

@@ -44,6 +44,19 @@ inline int launch_check(const char* name) {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Pointers read from memory (descriptor tables, structs) are generic to the
+// compiler and would use flat instructions; these casts assert they point to
+// device global memory so loads and stores become global_* instructions.
+#define TR_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const TR_GLOBAL T* gptr(const T* p) {
+  return (const TR_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ TR_GLOBAL T* gptr(T* p) {
+  return (TR_GLOBAL T*)p;
+}
+
 inline unsigned grid_for(int64_t n, int block = kBlock) {
   return unsigned((n + block - 1) / block);
 }

@@ -454,8 +454,12 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
     auto* s = new tr_sim();
     s->form = form;
     s->v = *view;
+    // a lone FP32 grid sizes its strips to fill the device by itself
+    tr_tuning t{};
+    if (tune) t = *tune;
+    if (t.strip_rows <= 0) t.strip_rows = tr_default_strip_rows(view->rows, view->cols);
     int rc = tr_batch_create(0, 0, &s->batch);
-    if (!rc) rc = tr_batch_add_ex(s->batch, view, tune, &s->member);
+    if (!rc) rc = tr_batch_add_ex(s->batch, view, &t, &s->member);
     if (rc) {
       delete s;
       return rc;

@@ -39,118 +39,154 @@ constexpr float kSqrtGf = 3.132091953f;
 constexpr float kMaxDtF = 1e10f;
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1/sqrt(x) for x >= 0; subnormal inputs (depths at wetting fronts) are
+// raised to 1e-30 first because the hardware rsq returns +inf for them, and
+// h*rsq(h) must stay finite.
+__device__ __forceinline__ float rsqf(float x) { return __builtin_amdgcn_rsqf(fmaxf(x, 1e-30f)); }
 __device__ __forceinline__ float sqrtf_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 struct Side {
   float h, qn, qt, un, ut, sq;
 };
 
-// One face in normal/tangential components.  Outputs the fluctuations sent to
-// the cell behind the face (l*) and in front of it (r*): mass, normal and
-// tangential momentum, before the dt/dx scaling.
+// One face in normal/tangential components: fluctuations sent to the cell
+// behind the face (l*) and in front of it (r*) for mass, normal and
+// tangential momentum, before the dt/dx scaling.  Same scheme as
+// src/kernels_flux.h:81-265, arranged for FP32 throughput:
+//  * 1/c and c from one reciprocal square root;
+//  * the wave-1 part reduces to ub*(dqt - vb*dh) on the tangential momentum;
+//  * the dry-side tests hp - b0/l < -eps are evaluated as sign tests of
+//    ((hp+eps)*l - b0)*l, without the division;
+//  * both wet-wet clip bounds share one reciprocal;
+//  * the Harten-Hyman entropy fix runs only in waves where some lane needs it;
+//  * the front side is the total fluctuation minus the back side;
+//  * no final flush (the threshold's job is the dry tests above).
 __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, float hdxdt,
-                                       float kEpsF, float& lm, float& ln, float& lt, float& rm,
+                                       float eps, float& lm, float& ln, float& lt, float& rm,
                                        float& rn, float& rt) {
-  lm = ln = lt = rm = rn = rt = 0.f;
-  if (!(L.h > 0.f || R.h > 0.f)) return;
-  const float c = sqrtf_hw(kGf * (0.5f * (L.h + R.h)));
-  const float ic = rcpf(c);
+  const bool active = L.h > 0.f || R.h > 0.f;
+  const float g2 = (0.5f * kGf) * (L.h + R.h);
+  const float ic = rsqf(g2);
+  const float c = g2 * ic;
   const float is = rcpf(L.sq + R.sq);
   const float ub = (L.un * L.sq + R.un * R.sq) * is;
   const float vb = (L.ut * L.sq + R.ut * R.sq) * is;
   float l0 = ub - c, l2 = ub + c, e0 = 0.f, e2 = 0.f;
-  {
-    const float a = L.un - kSqrtGf * L.sq, b = R.un - kSqrtGf * R.sq;
-    if (a < 0.f && b > 0.f) {
-      const float sp = a * (b - l0) * rcpf(b - a);
-      e0 = l0 - sp;
-      l0 = sp;
-    }
-  }
-  {
-    const float a = L.un + kSqrtGf * L.sq, b = R.un + kSqrtGf * R.sq;
-    if (a < 0.f && b > 0.f) {
-      const float sp = b * (l2 - a) * rcpf(b - a);
-      e2 = l2 - sp;
-      l2 = sp;
-    }
+  const float aL0 = L.un - kSqrtGf * L.sq, aR0 = R.un - kSqrtGf * R.sq;
+  const float aL2 = L.un + kSqrtGf * L.sq, aR2 = R.un + kSqrtGf * R.sq;
+  const bool fix0 = aL0 < 0.f && aR0 > 0.f;
+  const bool fix2 = aL2 < 0.f && aR2 > 0.f;
+  if (__any(active && (fix0 || fix2))) {
+    const float sp0 = aL0 * (aR0 - l0) * rcpf(aR0 - aL0);
+    const float sp2 = aR2 * (l2 - aL2) * rcpf(aR2 - aL2);
+    e0 = fix0 ? l0 - sp0 : 0.f;
+    l0 = fix0 ? sp0 : l0;
+    e2 = fix2 ? l2 - sp2 : 0.f;
+    l2 = fix2 ? sp2 : l2;
   }
   const float dh = R.h - L.h;
   const float w = ((R.qn - L.qn) - ub * dh) * ic;
   const float a0 = 0.5f * (dh - w), a2 = 0.5f * (dh + w);
-  // wave 1 only moves tangential momentum: alpha1 * eigenvector = dqt - vbar dh
   const float t1 = ub * ((R.qt - L.qt) - vb * dh);
-  float hp, dzp = dz;
-  if (dz >= 0.f) {
-    hp = L.h;
-    if (L.h < dz) dzp = L.h;
-  } else {
-    hp = R.h;
-    if (R.h < -dz) dzp = -R.h;
-  }
-  float b0 = 0.5f * kGf * (hp - 0.5f * fabsf(dzp)) * dzp * ic;
-  hp = L.h + a0;
-  if (l0 * l2 < 0.f && hp > 0.f && L.h > 0.f && R.h > 0.f) {
-    b0 = fmaxf(b0, l0 * rcpf(l0 + e2) * (a0 * l0 - L.h * hdxdt + e2 * R.h));
-    b0 = fminf(b0, l2 * rcpf(l2 + e0) * (-a2 * l2 + R.h * hdxdt + e0 * L.h));
+  // bed-slope source with the dry-step limit
+  const bool up = dz >= 0.f;
+  const float hp0 = up ? L.h : R.h;
+  const float adz = fabsf(dz);
+  const float dzp = hp0 < adz ? (up ? hp0 : -hp0) : dz;
+  float b0 = (0.5f * kGf) * (hp0 - 0.5f * fabsf(dzp)) * dzp * ic;
+  const float hp = L.h + a0;
+  const bool sub = l0 * l2 < 0.f && hp > 0.f && L.h > 0.f && R.h > 0.f;
+  if (__any(active && sub)) {
+    const float d0 = l0 + e2, d2 = l2 + e0;
+    const float rr = rcpf(d0 * d2);
+    const float lo = l0 * d2 * rr * (a0 * l0 - L.h * hdxdt + e2 * R.h);
+    const float hi = l2 * d0 * rr * (-a2 * l2 + R.h * hdxdt + e0 * L.h);
+    const float bc = fminf(fmaxf(b0, lo), hi);
+    b0 = sub ? bc : b0;
   }
   const float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
-  const float sL = hp - b0 * rcpf(l0), sR = hp - b0 * rcpf(l2);
   const float n0 = ub - c, n2 = ub + c;
-  if (sR < -kEpsF && R.h < kEpsF) {
-    lm = W0 + W2;
-  } else if (sL < -kEpsF && L.h < kEpsF) {
-    rm = W0 + W2;
-  } else {
-    if (l0 > 0.f) { rm += W0; rn += W0 * n0; rt += W0 * vb; }
-    else          { lm += W0; ln += W0 * n0; lt += W0 * vb; }
-    if (ub > 0.f) rt += t1;
-    else          lt += t1;
-    if (l2 > 0.f) { rm += W2; rn += W2 * n2; rt += W2 * vb; }
-    else          { lm += W2; ln += W2 * n2; lt += W2 * vb; }
+  const float he = hp + eps;
+  const bool to_left = ((he * l2 - b0) * l2 < 0.f) && R.h < eps;
+  const bool to_right = !to_left && ((he * l0 - b0) * l0 < 0.f) && L.h < eps;
+  // regular split: back side gets the waves with non-positive speed
+  const float B0 = l0 > 0.f ? 0.f : W0, B2 = l2 > 0.f ? 0.f : W2;
+  float bm = B0 + B2;
+  float bn = B0 * n0 + B2 * n2;
+  float bt = (B0 + B2) * vb + (ub > 0.f ? 0.f : t1);
+  const float tm = W0 + W2;
+  float tn = W0 * n0 + W2 * n2;
+  float tt = tm * vb + t1;
+  if (to_left || to_right) {
+    // all mass to the wet side, no momentum from the regular waves
+    bm = to_left ? tm : 0.f;
+    bn = 0.f;
+    bt = 0.f;
+    tn = 0.f;
+    tt = 0.f;
   }
-  {
+  float fm = tm - bm, fn = tn - bn, ft = tt - bt;
+  if (__any(active && (fix0 || fix2))) {
     const float q0 = e0 * a0, q2 = e2 * a2;
-    if (e0 > 0.f) { rm += q0; rn += q0 * n0; rt += q0 * vb; }
-    else          { lm += q0; ln += q0 * n0; lt += q0 * vb; }
-    if (e2 > 0.f) { rm += q2; rn += q2 * n2; rt += q2 * vb; }
-    else          { lm += q2; ln += q2 * n2; lt += q2 * vb; }
+    const float r0 = e0 > 0.f ? q0 : 0.f, r2 = e2 > 0.f ? q2 : 0.f;
+    const float k0 = q0 - r0, k2 = q2 - r2;
+    bm += k0 + k2;
+    bn += k0 * n0 + k2 * n2;
+    bt += (k0 + k2) * vb;
+    fm += r0 + r2;
+    fn += r0 * n0 + r2 * n2;
+    ft += (r0 + r2) * vb;
   }
-  lm = fabsf(lm) < kEpsF ? 0.f : lm;
-  ln = fabsf(ln) < kEpsF ? 0.f : ln;
-  lt = fabsf(lt) < kEpsF ? 0.f : lt;
-  rm = fabsf(rm) < kEpsF ? 0.f : rm;
-  rn = fabsf(rn) < kEpsF ? 0.f : rn;
-  rt = fabsf(rt) < kEpsF ? 0.f : rt;
+  lm = active ? bm : 0.f;
+  ln = active ? bn : 0.f;
+  lt = active ? bt : 0.f;
+  rm = active ? fm : 0.f;
+  rn = active ? fn : 0.f;
+  rt = active ? ft : 0.f;
 }
 
 __device__ __forceinline__ float cell_dt32(float h, float qx, float qy, float cn_dx, float hextra) {
   if (h > hextra) {
-    const float umax = fmaxf(fabsf(qx), fabsf(qy)) * rcpf(h);
-    return cn_dx * rcpf(umax + kSqrtGf * sqrtf_hw(h));
+    const float r = rsqf(h);
+    const float umax = fmaxf(fabsf(qx), fabsf(qy)) * (r * r);
+    return cn_dx * rcpf(umax + kSqrtGf * (h * r));
   }
   return kMaxDtF;
 }
 
-// Row of input state with its derived per-cell quantities.
-struct Row {
+// The member's fields the step reads, copied to registers once.
+struct Loc {
+  const TR_GLOBAL float* __restrict__ h;
+  const TR_GLOBAL float* __restrict__ qx;
+  const TR_GLOBAL float* __restrict__ qy;
+  const TR_GLOBAL float* __restrict__ dzx;
+  const TR_GLOBAL float* __restrict__ dzy;
+  const TR_GLOBAL float* __restrict__ n2;
+  const TR_GLOBAL int* __restrict__ rid;
+  const TR_GLOBAL double* __restrict__ zadd;
+  int rows, cols, nz;
+  float add0;
+};
+
+struct Raw {
   float h, qx, qy, dzx, dzy, n2, add;
 };
 
-__device__ __forceinline__ Row load_row(const InstF32& P, int cur, int r, int c, bool col_ok,
-                                        bool col_int, float add0) {
-  Row w{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (r >= 0 && r < P.rows && col_ok) {
-    const long long i = (long long)r * P.cols + c;
-    w.h = P.h[cur][i];
-    w.qx = P.qx[cur][i];
-    w.qy = P.qy[cur][i];
-    w.dzx = P.dzx[i];
-    w.dzy = P.dzy[i];
-    if (P.nman2) w.n2 = P.nman2[i];
-    if (P.num_runoffs > 0 && r >= 1 && r <= P.rows - 2 && col_int)
-      w.add = P.runoff_id ? float(P.zone_add[cur * P.num_runoffs + P.runoff_id[i]]) : add0;
-  }
+// Row r, column c of the input, with r and c clamped into the grid: rows and
+// columns outside it only feed halo or ghost cells that are never updated,
+// so they may hold copies instead of zeros, and the loads need no branches.
+__device__ __forceinline__ Raw load_row(const Loc& L, int r, int c) {
+  const int rr = min(max(r, 0), L.rows - 1);
+  const int cc = min(max(c, 0), L.cols - 1);
+  const long long i = (long long)rr * L.cols + cc;
+  Raw w;
+  w.h = L.h[i];
+  w.qx = L.qx[i];
+  w.qy = L.qy[i];
+  w.dzx = L.dzx[i];
+  w.dzy = L.dzy[i];
+  w.n2 = L.n2 ? L.n2[i] : 0.f;
+  w.add = L.nz > 0 ? (L.rid ? float(L.zadd[L.rid[i]]) : L.add0) : 0.f;
   return w;
 }
 
@@ -167,55 +203,73 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
   const float hdxdt = float(0.5 * P.dx / dt64);
   const float hextra = P.hextra_f;
   const float cn_dx = P.cn_dx_f;
-  const float kEpsF = P.eps_f;
-  const double* __restrict__ src_add = P.src_add + cur * P.nsrc;
-  // rain depth of this step (FP64 product rate*dt, rounded once); zone 0
-  // when the member has one uniform zone
-  const float add0 = P.num_runoffs > 0 ? float(P.zone_add[cur * P.num_runoffs]) : 0.f;
+  const float eps = P.eps_f;
   const bool open_bc = P.open_bc != 0;
   const int R = P.rows, Cc = P.cols;
-  const int bx = wk.y % P.nbx, by = wk.y / P.nbx;
-  float* __restrict__ ho = P.h[cur ^ 1];
-  float* __restrict__ qxo = P.qx[cur ^ 1];
-  float* __restrict__ qyo = P.qy[cur ^ 1];
+  const int nsrc = P.nsrc;
+  const TR_GLOBAL int* __restrict__ src_row = gptr(P.src_row);
+  const TR_GLOBAL int* __restrict__ src_col = gptr(P.src_col);
+  const TR_GLOBAL double* __restrict__ src_add = gptr(P.src_add + cur * nsrc);
+  TR_GLOBAL float* __restrict__ ho = gptr(P.h[cur ^ 1]);
+  TR_GLOBAL float* __restrict__ qxo = gptr(P.qx[cur ^ 1]);
+  TR_GLOBAL float* __restrict__ qyo = gptr(P.qy[cur ^ 1]);
+  TR_GLOBAL float* __restrict__ maxh = gptr(P.maxh);
+  Loc L;
+  L.h = gptr(P.h[cur]);
+  L.qx = gptr(P.qx[cur]);
+  L.qy = gptr(P.qy[cur]);
+  L.dzx = gptr(P.dzx);
+  L.dzy = gptr(P.dzy);
+  L.n2 = gptr(P.nman2);
+  L.rid = gptr(P.runoff_id);
+  L.nz = P.num_runoffs;
+  L.zadd = gptr(P.num_runoffs > 0 ? P.zone_add + cur * P.num_runoffs : nullptr);
+  L.rows = R;
+  L.cols = Cc;
+  // rain depth of this step (FP64 product rate*dt, rounded once)
+  L.add0 = P.num_runoffs > 0 ? float(L.zadd[0]) : 0.f;
+  const int nbx = P.nbx, nby = P.nby, strip = P.strip_rows;
+  const unsigned nblocks = P.nblocks;
+  const int bx = wk.y % nbx, by = wk.y / nbx;
 
   const int c = kOutCols * bx - 1 + lane;
-  const bool col_ok = c >= 0 && c < Cc;
   const bool col_int = c >= 1 && c <= Cc - 2;
-  const bool xface_ok = c >= 0 && c <= Cc - 2 && lane < kLanes - 1;
   const bool out_lane = lane >= 2 && lane <= kOutCols + 1 && col_int;
   const bool ghost_w = c == 0 && lane == 1;
   const bool ghost_e = c == Cc - 1 && lane >= 3 && lane <= kLanes - 2;
   const bool col_mine = out_lane || ghost_w || ghost_e;
 
-  const int e_lo = 1 + P.strip_rows * by;
-  const int e_hi = min(e_lo + P.strip_rows - 1, R - 2);
+  const int e_lo = 1 + strip * by;
+  const int e_hi = min(e_lo + strip - 1, R - 2);
   const int e_min = by == 0 ? 0 : e_lo;
-  const int e_max = by == P.nby - 1 ? R - 1 : e_hi;
+  const int e_max = by == nby - 1 ? R - 1 : e_hi;
 
-  // previous row (input state and derived values)
+  // previous row: input state, derived values, and its RHS sums so far
   float ph = 0.f, pqx = 0.f, pqy = 0.f, pu = 0.f, pv = 0.f, psq = 0.f, pn2 = 0.f, padd = 0.f;
-  float pdzy = 0.f, pdzx = 0.f;
-  float px0h = 0.f, px0x = 0.f, px0y = 0.f, px1h = 0.f, px1x = 0.f, px1y = 0.f;
-  float py0h = 0.f, py0x = 0.f, py0y = 0.f;
+  float pdzx = 0.f, pdzy = 0.f;
+  float p0h = 0.f, p0x = 0.f, p0y = 0.f, p1h = 0.f, p1x = 0.f, p1y = 0.f;
   // updated rows r-2 (A) and r-3 (B)
   float ah = 0.f, aqx = 0.f, aqy = 0.f, adzx = 0.f, adzy = 0.f;
   float bh = 0.f;
   float best = kMaxDtF * 2.f;
+  int ks = 0;  // first source not above the row being updated
 
-  Row nxt = load_row(P, cur, e_min - 2, c, col_ok, col_int, add0);
+  Raw nxt = load_row(L, e_min - 2, c);
   for (int r = e_min - 2; r <= e_max + 2; ++r) {
-    const Row s = nxt;
-    if (r + 1 <= e_max + 2) nxt = load_row(P, cur, r + 1, c, col_ok, col_int, add0);
-    const bool row_ok = r >= 0 && r < R;
-    const float sq = s.h > 0.f ? sqrtf_hw(s.h) : 0.f;
-    const float ih = rcpf(s.h);
+    const Raw s = nxt;
+    if (r + 1 <= e_max + 2) nxt = load_row(L, r + 1, c);
+    const float rs = rsqf(s.h);
+    const float sq = s.h > 0.f ? s.h * rs : 0.f;
+    const float ih = rs * rs;
     const bool mv = s.h >= hextra;
     const float u = mv ? s.qx * ih : 0.f;
     const float v = mv ? s.qy * ih : 0.f;
 
-    // x face (c, c+1) of row r
-    float x0h = 0.f, x0x = 0.f, x0y = 0.f, x1h, x1x, x1y;
+    // x face (c, c+1) of row r: back side stays in this lane (rhs*0), the
+    // front side goes to lane c+1 (rhs*1).  Faces outside the reference's
+    // ranges only feed ghost or halo cells that are never updated, so they
+    // are computed unmasked; fluctuations are scaled by dt/dx at the update.
+    float c0h, c0x, c0y, c1h, c1x, c1y;
     {
       Side Rt;
       Rt.h = __shfl_down(s.h, 1);
@@ -224,32 +278,25 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
       Rt.un = __shfl_down(u, 1);
       Rt.ut = __shfl_down(v, 1);
       Rt.sq = __shfl_down(sq, 1);
-      float em = 0.f, en = 0.f, et = 0.f;
-      if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
-        const Side Lt{s.h, s.qx, s.qy, u, v, sq};
-        float lm, ln, lt;
-        face32(Lt, Rt, s.dzx, hdxdt, kEpsF, lm, ln, lt, em, en, et);
-        x0h = lm * dtdx;
-        x0x = ln * dtdx;
-        x0y = lt * dtdx;
-        em *= dtdx;
-        en *= dtdx;
-        et *= dtdx;
-      }
-      x1h = __shfl_up(em, 1);
-      x1x = __shfl_up(en, 1);
-      x1y = __shfl_up(et, 1);
+      const Side Lt{s.h, s.qx, s.qy, u, v, sq};
+      float em, en, et;
+      face32(Lt, Rt, s.dzx, hdxdt, eps, c0h, c0x, c0y, em, en, et);
+      c1h = __shfl_up(em, 1);
+      c1x = __shfl_up(en, 1);
+      c1y = __shfl_up(et, 1);
     }
 
-    // y face between row r (south) and row r-1 (north): normal = y
-    float y0h = 0.f, y0x = 0.f, y0y = 0.f, y1h = 0.f, y1x = 0.f, y1y = 0.f;
-    if (row_ok && r >= 1 && col_int) {
+    // y face between row r (south, back) and row r-1 (north, front)
+    float y1h, y1x, y1y;
+    {
       const Side S{s.h, s.qy, s.qx, v, u, sq};
       const Side N{ph, pqy, pqx, pv, pu, psq};
-      float lm, ln, lt, rm, rn, rt;
-      face32(S, N, s.dzy, hdxdt, kEpsF, lm, ln, lt, rm, rn, rt);
-      y0h = lm * dtdx; y0y = ln * dtdx; y0x = lt * dtdx;
-      y1h = rm * dtdx; y1y = rn * dtdx; y1x = rt * dtdx;
+      float lm, ln, lt;
+      face32(S, N, s.dzy, hdxdt, eps, lm, ln, lt, y1h, y1y, y1x);
+      // rhs*0 of row r = x back side + y back side (reference order)
+      c0h += lm;
+      c0y += ln;
+      c0x += lt;
     }
 
     // update row r-1
@@ -257,40 +304,36 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
     float nh = ph, nqx = pqx, nqy = pqy;
     const bool ru_int = ru >= 1 && ru <= R - 2;
     if (ru_int && col_int) {
-      float hnew = ph - (px0h + py0h) - (px1h + y1h);
+      float hnew = ph - dtdx * (p0h + (p1h + y1h));
       hnew = hnew < 0.f ? 0.f : hnew;
-      if (hnew < hextra) {
-        nqx = 0.f;
-        nqy = 0.f;
-      } else {
-        const float mx = pqx - (px0x + py0x) - (px1x + y1x);
-        const float my = pqy - (px0y + py0y) - (px1y + y1y);
-        const float mm = sqrtf_hw(mx * mx + my * my);
-        if (pn2 > kEpsF && ph >= hextra && mm > kEpsF) {
-          // h^(7/3) = h*h*cbrt(h)
-          const float h73 = __builtin_amdgcn_exp2f(2.3333333f * __builtin_amdgcn_logf(ph));
-          const float tt = dt * kGf * pn2 * mm * rcpf(h73);
-          const float k = 2.f * rcpf(1.f + sqrtf_hw(1.f + 4.f * tt));
-          nqx = mx * k;
-          nqy = my * k;
-        } else {
-          nqx = mx;
-          nqy = my;
-        }
+      const float mx = pqx - dtdx * (p0x + (p1x + y1x));
+      const float my = pqy - dtdx * (p0y + (p1y + y1y));
+      float kf = 1.f;
+      if (pn2 > eps && ph >= hextra) {
+        const float m2 = mx * mx + my * my;
+        const float mm = sqrtf_hw(m2);
+        // tt = dt g n^2 |m| / h^(7/3); q = m * 2/(1+sqrt(1+4tt))
+        const float h73 = __builtin_amdgcn_exp2f(2.3333333f * __builtin_amdgcn_logf(ph));
+        const float tt = dt * kGf * pn2 * mm * rcpf(h73);
+        const float k = 2.f * rcpf(1.f + sqrtf_hw(1.f + 4.f * tt));
+        kf = mm > eps ? k : 1.f;
       }
-      if (P.num_runoffs > 0) {
+      const bool wet = hnew >= hextra;
+      nqx = wet ? mx * kf : 0.f;
+      nqy = wet ? my * kf : 0.f;
+      if (L.nz > 0) {
         hnew += padd;
-        if (hnew < hextra) {
-          hnew = hnew < 0.f ? 0.f : hnew;
-          nqx = 0.f;
-          nqy = 0.f;
-        }
+        const bool low = hnew < hextra;
+        hnew = hnew < 0.f ? 0.f : hnew;
+        nqx = low ? 0.f : nqx;
+        nqy = low ? 0.f : nqy;
       }
       nh = hnew;
     }
-    if (ru_int) {
-      for (int k = 0; k < P.nsrc; k++)
-        if (P.src_row[k] == ru && P.src_col[k] == c) nh = nh + float(src_add[k]);
+    if (nsrc > 0 && ru_int) {
+      while (ks < nsrc && src_row[ks] < ru) ks++;
+      for (int k = ks; k < nsrc && src_row[k] == ru; k++)
+        if (src_col[k] == c) nh = nh + float(src_add[k]);
     }
     if (open_bc) {
       const float rh = __shfl_down(nh, 1), rqx = __shfl_down(nqx, 1), rqy = __shfl_down(nqy, 1);
@@ -310,15 +353,15 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
       const float dzW = -__shfl_up(adzx, 1);
       const bool cell_int = e >= 1 && e <= R - 2 && col_int;
       if (cell_int && ah > hextra) {
-        if ((ah < adzx && hE < kEpsF) || (ah < dzW && hW < kEpsF)) qxq = 0.f;
-        if ((ah < adzy && bh < kEpsF) || (ah < -pdzy && nh < kEpsF)) qyq = 0.f;
+        if ((ah < adzx && hE < eps) || (ah < dzW && hW < eps)) qxq = 0.f;
+        if ((ah < adzy && bh < eps) || (ah < -pdzy && nh < eps)) qyq = 0.f;
       }
       if (col_mine) {
         const long long o = (long long)e * Cc + c;
         ho[o] = ah;
         qxo[o] = qxq;
         qyo[o] = qyq;
-        if (P.maxh && cell_int && ah > P.maxh[o]) P.maxh[o] = ah;
+        if (maxh && cell_int && ah > maxh[o]) maxh[o] = ah;
         best = fminf(best, cell_dt32(ah, qxq, qyq, cn_dx, hextra));
       }
     }
@@ -328,8 +371,7 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
     ah = nh; aqx = nqx; aqy = nqy; adzx = pdzx; adzy = pdzy;
     ph = s.h; pqx = s.qx; pqy = s.qy; pu = u; pv = v; psq = sq; pn2 = s.n2; padd = s.add;
     pdzx = s.dzx; pdzy = s.dzy;
-    px0h = x0h; px0x = x0x; px0y = x0y; px1h = x1h; px1x = x1x; px1y = x1y;
-    py0h = y0h; py0x = y0x; py0y = y0y;
+    p0h = c0h; p0x = c0x; p0y = c0y; p1h = c1h; p1x = c1x; p1y = c1y;
   }
   // wave min of the float dt (positive floats order like their bit patterns)
   unsigned int bits = __float_as_uint(best);
@@ -338,6 +380,7 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
     bits = o < bits ? o : bits;
   }
   if (lane == 0) {
+    (void)nblocks;
     const float m = __uint_as_float(bits);
     const unsigned long long wb = m >= kMaxDtF * 2.f ? ~0ull : dbits(double(m));
     close_step_cursor(P, cur, wb, true);
@@ -378,14 +421,15 @@ __global__ __launch_bounds__(64) void k_prime_f32(const InstF32* inst, int membe
 
 int fused_grid_x_f32(int cols) { return (cols - 2 + kOutCols - 1) / kOutCols; }
 
-// Rows per wave: long strips amortise the two-row halo, but a grid must also
-// give the device enough waves (about 8 per CU) to hide memory latency.
+// Rows per wave for a grid stepped on its own: long strips amortise the
+// two-row halo and the per-wave set-up (measured best: 96 rows on a 4096^2
+// grid, about 3 waves per SIMD), short ones give a small grid enough waves
+// (16 rows on 1024^2).
 int default_strip_rows(int rows, int cols) {
   const int nbx = fused_grid_x_f32(cols);
-  const int target_waves = 2048;
-  int want_nby = (target_waves + nbx - 1) / nbx;
-  int strip = (rows - 2 + want_nby - 1) / want_nby;
-  return strip < 16 ? 16 : (strip > 64 ? 64 : strip);
+  const int64_t target_waves = 3072;
+  int64_t strip = (int64_t(rows - 2) * nbx + target_waves - 1) / target_waves;
+  return strip < 16 ? 16 : (strip > 128 ? 128 : int(strip));
 }
 
 int launch_step_f32(const BatchArgs& a, int nwork, hipStream_t s) {
