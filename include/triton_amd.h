@@ -163,10 +163,12 @@ typedef struct tr_sim_status {
 
 /* Tuning of one simulation / batch member (zero fields take the defaults). */
 typedef struct tr_tuning {
-  int32_t strip_rows;  /* rows per wave of the fused step (0: by grid size) */
-  double dry_eps;      /* FP32 form: dry / flush threshold (0: 1e-10; the
-                          reference's FP32 build uses 1e-6, src/constants.h:43,
-                          which loses mass on small-scale flows) */
+  int32_t strip_rows;      /* FP64 fused form: rows per wave (0: by grid size) */
+  double dry_eps;          /* FP32 form: dry / flush threshold (0: 1e-10; the
+                              reference's FP32 build uses 1e-6,
+                              src/constants.h:43, which loses mass on
+                              small-scale flows) */
+  int32_t waves_per_simd;  /* FP32 form: waves per SIMD of the launch (0: 3) */
 } tr_tuning;
 
 /* Allocate device state for `view` in the given form and upload it. */
@@ -207,13 +209,16 @@ int tr_dem_flow_route(const double *dem, int32_t nr, int32_t nc, double eps, dou
  * added and removed between launches; tr_batch_add fails with TR_ERR_NOMEM
  * when the member would exceed the HBM budget given at creation. */
 typedef struct tr_batch tr_batch;
-int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch **out);
+/* Every launch splits the rows of all active members evenly over
+ * waves_per_simd (0: 3) waves per SIMD of the device. */
+int tr_batch_create(int64_t hbm_budget_bytes, int32_t waves_per_simd, tr_batch **out);
 /* Device bytes a member built from `v` would hold (admission control). */
 int64_t tr_batch_member_bytes(const tr_case_view *v);
 int tr_batch_add(tr_batch *b, const tr_case_view *v, int32_t *id);
+/* tune->dry_eps applies per member; the other tuning fields are the batch's. */
 int tr_batch_add_ex(tr_batch *b, const tr_case_view *v, const tr_tuning *tune, int32_t *id);
-/* Default rows per wave for a padded rows x cols grid (enough waves to fill
- * the device; long strips otherwise). */
+/* Default rows per wave of the FP64 fused step for a padded rows x cols grid
+ * (enough waves to fill the device; long strips otherwise). */
 int32_t tr_default_strip_rows(int32_t rows, int32_t cols);
 int tr_batch_remove(tr_batch *b, int32_t id);
 /* Limits for the steps queued next, for every member: t_stop (<= 0: each

@@ -70,7 +70,7 @@ class CaseView(C.Structure):
 class Tuning(C.Structure):
     """tr_tuning: rows per wave of the fused step and the FP32 dry threshold."""
 
-    _fields_ = [("strip_rows", C.c_int32), ("dry_eps", C.c_double)]
+    _fields_ = [("strip_rows", C.c_int32), ("dry_eps", C.c_double), ("waves_per_simd", C.c_int32)]
 
 
 class SimStatus(C.Structure):
@@ -354,12 +354,12 @@ class Simulation:
     """A device-resident simulation of one case (tr_sim_*)."""
 
     def __init__(self, case: Case, form: int = FORM_F64_FUSED, strip_rows: int = 0,
-                 dry_eps: float = 0.0):
+                 dry_eps: float = 0.0, waves_per_simd: int = 0):
         self.case = case
         self.form = form
         self._view = case.view()
         h = C.c_void_p()
-        tune = Tuning(int(strip_rows), float(dry_eps))
+        tune = Tuning(int(strip_rows), float(dry_eps), int(waves_per_simd))
         check(lib().tr_sim_create_ex(C.byref(self._view), int(form), C.byref(tune), C.byref(h)))
         self._h = h
 
@@ -418,9 +418,9 @@ class Batch:
     (tr_batch_*).  Members keep their own dt, time and forcing; they may be
     added and removed between launches, within an HBM budget."""
 
-    def __init__(self, hbm_budget_bytes: int = 0, strip_rows: int = 0):
+    def __init__(self, hbm_budget_bytes: int = 0, waves_per_simd: int = 0):
         h = C.c_void_p()
-        check(lib().tr_batch_create(int(hbm_budget_bytes), int(strip_rows), C.byref(h)))
+        check(lib().tr_batch_create(int(hbm_budget_bytes), int(waves_per_simd), C.byref(h)))
         self._h = h
         self.members: dict[int, Case] = {}
         self._views: dict[int, CaseView] = {}
@@ -447,10 +447,10 @@ class Batch:
         v = case.view()
         return int(lib().tr_batch_member_bytes(C.byref(v)))
 
-    def add(self, case: Case, strip_rows: int = 0, dry_eps: float = 0.0) -> int:
+    def add(self, case: Case, dry_eps: float = 0.0) -> int:
         v = case.view()
         mid = C.c_int32()
-        tune = Tuning(int(strip_rows), float(dry_eps))
+        tune = Tuning(0, float(dry_eps), 0)
         check(lib().tr_batch_add_ex(self._h, C.byref(v), C.byref(tune), C.byref(mid)))
         self.members[mid.value] = case
         return mid.value

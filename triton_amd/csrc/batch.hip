@@ -1,8 +1,8 @@
 // batch.hip -- members of a batched FP32 run (tr_batch_*).
 //
 // Every member owns its device buffers and a control block pair with a
-// cursor; one launch per step covers all active members through a work
-// table of (member, wave) entries, so members of uneven size advance
+// cursor; one launch per step covers all active members through a table of
+// row ranges ("pieces") per wave, so members of uneven size advance
 // together while each keeps its own adaptive dt, time and forcing.  Members
 // can be added and removed between launches (the serving loop admits
 // requests this way); admission is checked against an HBM budget.
@@ -80,9 +80,10 @@ constexpr int kVolBlocks = 256;
 // 1e-10 kept the relative mass error at 5e-8 there.
 constexpr float kDefaultDryEpsF32 = 1e-10f;
 
-// Rows per wave for batch members (the longest strip measured fastest on a
-// full device).
-constexpr int kBatchStripRows = 96;
+// Waves per SIMD of a batched launch (the step kernel's registers allow 4;
+// 3 measured fastest) and the fewest rows a wave is given.
+constexpr int kWavesPerSimd = 3;
+constexpr int kMinRowsPerWave = 48;
 
 }  // namespace
 }  // namespace tr
@@ -92,17 +93,21 @@ using namespace tr;
 struct tr_batch {
   hipStream_t stream = nullptr;
   int64_t budget = 0;          // 0: no limit
-  int strip_rows = 64;
   std::vector<Member> m;
   int64_t bytes = 0;           // sum over members + shared
   InstF32* d_inst = nullptr;
   CtrlPair* d_ctrl = nullptr;
   CtrlPair* h_ctrl = nullptr;  // pinned mirror
   int cap = 0;
-  int2* d_work = nullptr;
-  int2* h_work = nullptr;      // pinned
-  int work_cap = 0;
-  int nwork = 0;
+  Piece* d_pieces = nullptr;
+  int* d_first = nullptr;
+  int* d_members = nullptr;    // [cap] active members, in partition order
+  int nmembers = 0;
+  int piece_cap = 0, first_cap = 0;
+  int nwork = 0;               // waves per launch
+  int npieces = 0;
+  int simd_count = 1024;
+  int waves_per_simd = kWavesPerSimd;
   bool table_dirty = true;
   double* d_part = nullptr;
   double* d_sum = nullptr;
@@ -114,11 +119,12 @@ struct tr_batch {
       for (void* p : mm.allocs) hipFree(p);
     hipFree(d_inst);
     hipFree(d_ctrl);
-    hipFree(d_work);
+    hipFree(d_pieces);
+    hipFree(d_first);
+    hipFree(d_members);
     hipFree(d_part);
     hipFree(d_sum);
     if (h_ctrl) hipHostFree(h_ctrl);
-    if (h_work) hipHostFree(h_work);
     if (h_sum) hipHostFree(h_sum);
     if (stream) hipStreamDestroy(stream);
   }
@@ -132,7 +138,11 @@ int grow(tr_batch* b, int need) {
   InstF32* di = nullptr;
   CtrlPair* dc = nullptr;
   CtrlPair* hc = nullptr;
+  int* dm = nullptr;
   TR_HIP(hipStreamSynchronize(b->stream));
+  TR_HIP(hipMalloc(reinterpret_cast<void**>(&dm), sizeof(int) * size_t(cap)));
+  hipFree(b->d_members);
+  b->d_members = dm;
   TR_HIP(hipMalloc(reinterpret_cast<void**>(&di), sizeof(InstF32) * size_t(cap)));
   TR_HIP(hipMalloc(reinterpret_cast<void**>(&dc), sizeof(CtrlPair) * size_t(cap)));
   TR_HIP(hipHostMalloc(reinterpret_cast<void**>(&hc), sizeof(CtrlPair) * size_t(cap)));
@@ -256,35 +266,82 @@ int64_t estimate_bytes(const tr_case_view& v, bool fric, bool zones) {
   return b + 4096;
 }
 
+// Partition every active member's rows over exactly waves_per_simd waves
+// per SIMD of the device, in equal shares, so no SIMD holds a wave more than
+// another (that one extra wave would set the step time).  The rows are laid
+// out band by band: each member is cut into row bands about one share high
+// and a band's column strips follow each other, so waves that run side by
+// side on one XCD stream the same rows of neighbouring strips and share
+// their halo columns in that XCD's L2.  A share that crosses a strip or band
+// boundary becomes two pieces.
 int rebuild_work(tr_batch* b) {
-  int total = 0;
-  for (auto& mm : b->m)
-    if (mm.used && mm.active) total += int(mm.inst.nblocks);
-  if (total > b->work_cap) {
-    TR_HIP(hipStreamSynchronize(b->stream));
-    hipFree(b->d_work);
-    if (b->h_work) hipHostFree(b->h_work);
-    b->d_work = nullptr;
-    b->h_work = nullptr;
-    int cap = std::max(total, 1024);
-    TR_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_work), sizeof(int2) * size_t(cap)));
-    TR_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_work), sizeof(int2) * size_t(cap)));
-    b->work_cap = cap;
-  }
   TR_HIP(hipStreamSynchronize(b->stream));
-  // largest members first: their long strips start earliest
   std::vector<int> order;
   for (int i = 0; i < int(b->m.size()); i++)
     if (b->m[size_t(i)].used && b->m[size_t(i)].active) order.push_back(i);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return b->m[size_t(x)].inst.nblocks > b->m[size_t(y)].inst.nblocks;
+    return b->m[size_t(x)].interior > b->m[size_t(y)].interior;
   });
-  int k = 0;
-  for (int i : order)
-    for (unsigned w = 0; w < b->m[size_t(i)].inst.nblocks; w++) b->h_work[k++] = make_int2(i, int(w));
-  b->nwork = k;
-  if (k) TR_HIP(hipMemcpyAsync(b->d_work, b->h_work, sizeof(int2) * size_t(k), hipMemcpyHostToDevice, b->stream));
-  TR_HIP(hipStreamSynchronize(b->stream));
+  int64_t total = 0;
+  for (int i : order) total += int64_t(b->m[size_t(i)].inst.nbx) * b->m[size_t(i)].inst.rows;
+  std::vector<Piece> pieces;
+  std::vector<int> first;
+  int nw = 0;
+  if (total > 0) {
+    const int64_t want = int64_t(b->simd_count) * b->waves_per_simd;
+    const int64_t by_rows = (total + kMinRowsPerWave - 1) / kMinRowsPerWave;
+    nw = int(std::max<int64_t>(1, std::min(want, by_rows)));
+    const int64_t share = (total + nw - 1) / nw;
+    struct Seg {
+      int member, bx, r0, n;
+    };
+    std::vector<Seg> segs;
+    for (int i : order) {
+      const InstF32& P = b->m[size_t(i)].inst;
+      const int nb = int((P.rows + share - 1) / share);
+      for (int k = 0; k < nb; k++) {
+        const int r0 = int(int64_t(P.rows) * k / nb), r1 = int(int64_t(P.rows) * (k + 1) / nb);
+        for (int bx = 0; bx < P.nbx; bx++) segs.push_back(Seg{i, bx, r0, r1 - r0});
+      }
+    }
+    size_t si = 0;
+    int off = 0;  // rows of segs[si] already handed out
+    for (int w = 0; w < nw; w++) {
+      first.push_back(int(pieces.size()));
+      int64_t take = (total * (w + 1)) / nw - (total * w) / nw;
+      while (take > 0 && si < segs.size()) {
+        const Seg& g = segs[si];
+        const int n = int(std::min<int64_t>(take, g.n - off));
+        pieces.push_back(Piece{g.member, g.bx, g.r0 + off, g.r0 + off + n - 1});
+        take -= n;
+        off += n;
+        if (off == g.n) {
+          off = 0;
+          si++;
+        }
+      }
+    }
+  }
+  first.push_back(int(pieces.size()));
+  const size_t np = std::max<size_t>(pieces.size(), 1), nf = first.size();
+  if (np > size_t(b->piece_cap) || nf > size_t(b->first_cap)) {
+    hipFree(b->d_pieces);
+    hipFree(b->d_first);
+    b->d_pieces = nullptr;
+    b->d_first = nullptr;
+    b->piece_cap = int(np * 2);
+    b->first_cap = int(nf * 2);
+    TR_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_pieces), sizeof(Piece) * size_t(b->piece_cap)));
+    TR_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_first), sizeof(int) * size_t(b->first_cap)));
+  }
+  if (!pieces.empty())
+    TR_HIP(hipMemcpy(b->d_pieces, pieces.data(), sizeof(Piece) * pieces.size(), hipMemcpyHostToDevice));
+  TR_HIP(hipMemcpy(b->d_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice));
+  if (!order.empty())
+    TR_HIP(hipMemcpy(b->d_members, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+  b->nmembers = int(order.size());
+  b->nwork = nw;
+  b->npieces = int(pieces.size());
   b->table_dirty = false;
   return TR_OK;
 }
@@ -325,15 +382,22 @@ Member* member(tr_batch* b, int id) {
 
 extern "C" {
 
-int tr_batch_create(int64_t hbm_budget_bytes, int32_t strip_rows, tr_batch** out) {
-  if (!out || hbm_budget_bytes < 0 || strip_rows < 0 || strip_rows > 4096) {
+int tr_batch_create(int64_t hbm_budget_bytes, int32_t waves_per_simd, tr_batch** out) {
+  if (!out || hbm_budget_bytes < 0 || waves_per_simd < 0 || waves_per_simd > 16) {
     set_error("tr_batch_create: bad arguments");
     return TR_ERR_INVALID;
   }
   *out = nullptr;
   auto* b = new tr_batch();
   b->budget = hbm_budget_bytes;
-  b->strip_rows = strip_rows;  // 0: kBatchStripRows unless a member says otherwise
+  if (waves_per_simd > 0) b->waves_per_simd = waves_per_simd;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      b->simd_count = prop.multiProcessorCount * 4;  // 4 SIMDs per CDNA compute unit
+  }
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     b->stream = nullptr;
@@ -399,13 +463,8 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   InstF32& P = mm.inst;
   P.rows = rows;
   P.cols = cols;
-  // members of a batch share the device, so each takes long strips
-  P.strip_rows = (tune && tune->strip_rows > 0) ? tune->strip_rows
-                 : (b->strip_rows > 0 ? b->strip_rows : kBatchStripRows);
   P.eps_f = (tune && tune->dry_eps > 0) ? float(tune->dry_eps) : kDefaultDryEpsF32;
   P.nbx = fused_grid_x_f32(cols);
-  P.nby = (rows - 2 + P.strip_rows - 1) / P.strip_rows;
-  P.nblocks = unsigned(P.nbx) * unsigned(P.nby);
   P.dx = v.dx;
   P.hextra = v.hextra;
   P.cn_dx = v.courant * v.dx;
@@ -586,13 +645,13 @@ int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
     return TR_ERR_INVALID;
   }
   if (b->table_dirty) RC(rebuild_work(b));
-  BatchArgs a{b->d_inst, b->d_work, b->nwork};
+  BatchArgs a{b->d_inst, b->d_pieces, b->d_first, b->nwork, b->d_members, b->nmembers};
   for (int64_t k = 0; k < nsteps; k++) RC(launch_step_f32(a, b->nwork, b->stream));
   return TR_OK;
 }
 
-// Wait for queued steps; members that reached their limit leave the work
-// table.  n_active gets the number of members still stepping.
+// Wait for queued steps; members that reached their limit leave the
+// partition.  n_active gets the number of members still stepping.
 int tr_batch_sync(tr_batch* b, int32_t* n_active) {
   if (!b) {
     set_error("tr_batch_sync: NULL batch");

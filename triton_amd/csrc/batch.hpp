@@ -1,8 +1,9 @@
 // batch.hpp -- descriptor of one member of a batched FP32 launch.
 //
-// A batched step launches one 64-lane wave per (member, strip); a small work
-// table maps the wave index to its member and strip, so members of any size
-// share one launch and each keeps its own dt, simulated time and forcing.
+// A batched step launches a fixed number of 64-lane waves; a piece table
+// hands each wave an equal share of the rows of all members' column strips,
+// so members of any size share one launch and each keeps its own dt,
+// simulated time and forcing.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,8 +16,7 @@ namespace tr {
 
 struct InstF32 {
   int rows, cols;            // padded grid
-  int strip_rows, nbx, nby;  // waves per member: nbx * nby
-  unsigned nblocks;
+  int nbx;                   // column strips of 60 interior columns
   double dx, hextra, cn_dx;  // FP64 copies for the step bookkeeping
   float dx_f, hextra_f, cn_dx_f;
   float eps_f;               // dry / flush threshold
@@ -49,13 +49,25 @@ struct InstF32 {
   int halt_on_print;
 };
 
-struct BatchArgs {
-  const InstF32* inst;
-  const int2* work;          // (member, wave index within the member)
-  int nwork;
+// A piece is a run of rows of one column strip of one member; waves take
+// equal numbers of rows, so a wave walks one or a few consecutive pieces.
+struct Piece {
+  int member;
+  int bx;          // column strip
+  int row_first;   // first and last row finalised (0 and rows-1 are ghosts)
+  int row_last;
 };
 
-int launch_step_f32(const BatchArgs& a, int nwork, hipStream_t s);
+struct BatchArgs {
+  const InstF32* inst;
+  const Piece* pieces;
+  const int* wave_first;     // pieces of wave w: [wave_first[w], wave_first[w+1])
+  int nwaves;
+  const int* members;        // members in the partition (the close kernel's list)
+  int nmembers;
+};
+
+int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s);
 int launch_prime_f32(const InstF32* d_inst, int member, const InstF32& host_inst, hipStream_t s);
 int fused_grid_x_f32(int cols);
 int default_strip_rows(int rows, int cols);

@@ -57,6 +57,46 @@ __device__ __forceinline__ TR_GLOBAL T* gptr(T* p) {
   return (TR_GLOBAL T*)p;
 }
 
+// Neighbour-lane moves across the whole 64-lane wave with DPP (wave_shl:1 /
+// wave_shr:1, GFX9-family controls): a VALU move instead of an LDS
+// permute.  Lane i receives lane i+1 (from_right) or lane i-1 (from_left);
+// the edge lane receives 0.  Callers use them in wave-uniform control flow.
+__device__ __forceinline__ int dpp_from_right(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_from_left(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float from_right(float x) {
+  return __int_as_float(dpp_from_right(__float_as_int(x)));
+}
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(dpp_from_left(__float_as_int(x)));
+}
+__device__ __forceinline__ double from_right(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_from_right(int(b & 0xffffffff));
+  const int hi = dpp_from_right(int(b >> 32));
+  return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+}
+__device__ __forceinline__ double from_left(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_from_left(int(b & 0xffffffff));
+  const int hi = dpp_from_left(int(b >> 32));
+  return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+}
+
+// Workgroups are handed to the 8 XCDs of the device round-robin (block b to
+// XCD b % 8).  xcd_slot(b, n) returns the logical index block b should work
+// on so that each XCD gets one contiguous range of the n logical tiles:
+// neighbouring strips then share their halo cache lines in one L2.
+constexpr int kNumXcd = 8;
+__host__ __device__ inline int xcd_slot(int b, int n) {
+  const int x = b % kNumXcd, k = b / kNumXcd;
+  const int base = n / kNumXcd, rem = n % kNumXcd;
+  return x * base + (x < rem ? x : rem) + k;
+}
+
 inline unsigned grid_for(int64_t n, int block = kBlock) {
   return unsigned((n + block - 1) / block);
 }

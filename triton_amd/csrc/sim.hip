@@ -326,7 +326,6 @@ int prime_fused(tr_sim* s) {
   c.simtime = v.sim_start;
   c.dt = 0.0;
   c.dtmin_bits = ~0ull;
-  c.arrivals = 0;
   c.halt = 0;
   c.step = 0;
   c.print_id = v.checkpoint_id;
@@ -454,11 +453,15 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
     auto* s = new tr_sim();
     s->form = form;
     s->v = *view;
-    // a lone FP32 grid sizes its strips to fill the device by itself
+    // a batch of one: its rows are split over the whole device
     tr_tuning t{};
     if (tune) t = *tune;
-    if (t.strip_rows <= 0) t.strip_rows = tr_default_strip_rows(view->rows, view->cols);
-    int rc = tr_batch_create(0, 0, &s->batch);
+    if (t.waves_per_simd < 0 || t.waves_per_simd > 16) {
+      delete s;
+      set_error("tr_sim_create_ex: waves_per_simd must be in [0, 16]");
+      return TR_ERR_INVALID;
+    }
+    int rc = tr_batch_create(0, t.waves_per_simd, &s->batch);
     if (!rc) rc = tr_batch_add_ex(s->batch, view, &t, &s->member);
     if (rc) {
       delete s;

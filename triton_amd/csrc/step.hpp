@@ -9,15 +9,15 @@
 
 namespace tr {
 
-// Control block of one step.  Two slots ping-pong: step n reads slot n&1,
-// reduces its per-cell dt into that slot and the last wave to finish writes
-// slot (n+1)&1 for the next step (the host-side loop of
+// Control block of one step.  Two slots ping-pong: step n reads slot n&1 and
+// reduces its per-cell dt into that slot; a one-thread close kernel then
+// writes slot (n+1)&1 for the next step (the host-side loop of
 // src/triton_simulate.h:54-134 and :202-226, moved on the device).
 struct alignas(128) StepCtrl {
   double simtime;                 // time at the start of this step
   double dt;                      // dt applied in this step (already clamped)
   unsigned long long dtmin_bits;  // min per-cell dt of this step's result
-  unsigned int arrivals;          // waves done with this step
+  int pad2;
   int halt;                       // nonzero: the step is skipped
   long long step;                 // steps completed before this one
   int print_id;

@@ -1,6 +1,5 @@
 // step_ctrl.hpp -- device routines that close a fused step: the wave-level
-// dt min with a single-pass "last wave" detection, and the single-thread
-// bookkeeping that the reference's host loop does between steps
+// dt min, and the single-thread bookkeeping that the reference's host loop does between steps
 // (src/triton_simulate.h:54-134, :202-226, :246-285).
 #pragma once
 
@@ -82,7 +81,6 @@ __device__ void prepare_next(const P& p, const StepCtrl& pv, StepCtrl& nx,
   nx.simtime = simtime;
   nx.dt = dt;
   nx.dtmin_bits = ~0ull;
-  nx.arrivals = 0u;
   nx.halt = halt ? 1 : 0;
   nx.step = step;
   nx.print_id = print_id;
@@ -102,45 +100,27 @@ __device__ void carry_halted(const P& p) {
   StepCtrl& nx = p.ctrl[p.cur ^ 1];
   nx = pv;
   nx.dtmin_bits = ~0ull;
-  nx.arrivals = 0u;
   for (int k = 0; k < p.nsrc; k++)
     p.src_add[(p.cur ^ 1) * p.nsrc + k] = p.src_add[p.cur * p.nsrc + k];
 }
 
-// Lane 0 of every wave: publish the wave's dt min, count arrivals and let the
-// last wave prepare the next step.
-template <class P>
-__device__ void close_step(const P& p, unsigned long long wave_min,
-                           bool after_step, unsigned nwaves) {
-  StepCtrl* cc = p.ctrl + p.cur;
+// Lane 0 of a wave publishes its dt min with a relaxed device-scope atomic.
+// The step kernel does nothing else with the control block: the close kernel
+// launched after it (stream order makes every min visible) prepares the next
+// slot.  A grid-wide "last wave" inside the step kernel would need a
+// device-scope fence per wave, which on this multi-die part writes back and
+// invalidates the die's L2 each time.
+__device__ __forceinline__ void publish_min(StepCtrl* cc, unsigned long long wave_min) {
   if (wave_min != ~0ull) atomicMin(&cc->dtmin_bits, wave_min);
-  __threadfence();
-  unsigned prev = atomicAdd(&cc->arrivals, 1u);
-  if (prev == nwaves - 1) {
-    __threadfence();
-    unsigned long long m = atomicMin(&cc->dtmin_bits, ~0ull);
-    prepare_next(p, *cc, p.ctrl[p.cur ^ 1], m, after_step, p.cur ^ 1);
-  }
 }
 
-// Batched instances keep a device-side cursor: every wave of a step reads
-// `*cursor` when it starts, so the last wave to arrive (all others have read
-// it) can prepare the other slot and flip the cursor.  Halted instances are
-// simply skipped, with nothing to carry.
+// Close kernel body for one instance (single thread): prepare the other slot
+// from this step's control block.  Halted steps carry the block unchanged
+// when the slot is host-driven (carry) or leave the cursor where it is.
 template <class P>
-__device__ void close_step_cursor(const P& p, int cur, unsigned long long wave_min,
-                                  bool after_step) {
-  StepCtrl* cc = p.ctrl + cur;
-  if (wave_min != ~0ull) atomicMin(&cc->dtmin_bits, wave_min);
-  __threadfence();
-  unsigned prev = atomicAdd(&cc->arrivals, 1u);
-  if (prev == p.nblocks - 1) {
-    __threadfence();
-    unsigned long long m = atomicMin(&cc->dtmin_bits, ~0ull);
-    prepare_next(p, *cc, p.ctrl[cur ^ 1], m, after_step, cur ^ 1);
-    __threadfence();
-    *p.cursor = cur ^ 1;
-  }
+__device__ void close_one(const P& p, int cur, bool after_step) {
+  const StepCtrl& cc = p.ctrl[cur];
+  prepare_next(p, cc, p.ctrl[cur ^ 1], cc.dtmin_bits, after_step, cur ^ 1);
 }
 
 }  // namespace tr

@@ -190,13 +190,11 @@ __device__ __forceinline__ Raw load_row(const Loc& L, int r, int c) {
   return w;
 }
 
-__global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
-  const int2 wk = a.work[blockIdx.x];
-  const InstF32& P = a.inst[wk.x];
-  const int cur = *P.cursor;
+// One member's piece: rows [row_first, row_last] of column strip bx, with
+// the two-row halo recomputed at its ends.  Returns the piece's dt min bits.
+__device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, int e_min, int e_max,
+                                            int lane) {
   const StepCtrl& CC = P.ctrl[cur];
-  if (CC.halt) return;
-  const int lane = threadIdx.x;
   const double dt64 = CC.dt;
   const float dt = float(dt64);
   const float dtdx = float(dt64 / P.dx);
@@ -228,9 +226,6 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
   L.cols = Cc;
   // rain depth of this step (FP64 product rate*dt, rounded once)
   L.add0 = P.num_runoffs > 0 ? float(L.zadd[0]) : 0.f;
-  const int nbx = P.nbx, nby = P.nby, strip = P.strip_rows;
-  const unsigned nblocks = P.nblocks;
-  const int bx = wk.y % nbx, by = wk.y / nbx;
 
   const int c = kOutCols * bx - 1 + lane;
   const bool col_int = c >= 1 && c <= Cc - 2;
@@ -238,11 +233,6 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
   const bool ghost_w = c == 0 && lane == 1;
   const bool ghost_e = c == Cc - 1 && lane >= 3 && lane <= kLanes - 2;
   const bool col_mine = out_lane || ghost_w || ghost_e;
-
-  const int e_lo = 1 + strip * by;
-  const int e_hi = min(e_lo + strip - 1, R - 2);
-  const int e_min = by == 0 ? 0 : e_lo;
-  const int e_max = by == nby - 1 ? R - 1 : e_hi;
 
   // previous row: input state, derived values, and its RHS sums so far
   float ph = 0.f, pqx = 0.f, pqy = 0.f, pu = 0.f, pv = 0.f, psq = 0.f, pn2 = 0.f, padd = 0.f;
@@ -272,18 +262,18 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
     float c0h, c0x, c0y, c1h, c1x, c1y;
     {
       Side Rt;
-      Rt.h = __shfl_down(s.h, 1);
-      Rt.qn = __shfl_down(s.qx, 1);
-      Rt.qt = __shfl_down(s.qy, 1);
-      Rt.un = __shfl_down(u, 1);
-      Rt.ut = __shfl_down(v, 1);
-      Rt.sq = __shfl_down(sq, 1);
+      Rt.h = from_right(s.h);
+      Rt.qn = from_right(s.qx);
+      Rt.qt = from_right(s.qy);
+      Rt.un = from_right(u);
+      Rt.ut = from_right(v);
+      Rt.sq = from_right(sq);
       const Side Lt{s.h, s.qx, s.qy, u, v, sq};
       float em, en, et;
       face32(Lt, Rt, s.dzx, hdxdt, eps, c0h, c0x, c0y, em, en, et);
-      c1h = __shfl_up(em, 1);
-      c1x = __shfl_up(en, 1);
-      c1y = __shfl_up(et, 1);
+      c1h = from_left(em);
+      c1x = from_left(en);
+      c1y = from_left(et);
     }
 
     // y face between row r (south, back) and row r-1 (north, front)
@@ -336,8 +326,8 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
         if (src_col[k] == c) nh = nh + float(src_add[k]);
     }
     if (open_bc) {
-      const float rh = __shfl_down(nh, 1), rqx = __shfl_down(nqx, 1), rqy = __shfl_down(nqy, 1);
-      const float lh = __shfl_up(nh, 1), lqx = __shfl_up(nqx, 1), lqy = __shfl_up(nqy, 1);
+      const float rh = from_right(nh), rqx = from_right(nqx), rqy = from_right(nqy);
+      const float lh = from_left(nh), lqx = from_left(nqx), lqy = from_left(nqy);
       if (ru_int && c == 0) { nh = rh; nqx = rqx; nqy = rqy; }
       if (ru_int && c == Cc - 1) { nh = lh; nqx = lqx; nqy = lqy; }
       if (ru == 1) { ah = nh; aqx = nqx; aqy = nqy; }
@@ -348,9 +338,9 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
     const int e = r - 2;
     if (e >= e_min && e <= e_max && e >= 0) {
       float qxq = aqx, qyq = aqy;
-      const float hE = __shfl_down(ah, 1);
-      const float hW = __shfl_up(ah, 1);
-      const float dzW = -__shfl_up(adzx, 1);
+      const float hE = from_right(ah);
+      const float hW = from_left(ah);
+      const float dzW = -from_left(adzx);
       const bool cell_int = e >= 1 && e <= R - 2 && col_int;
       if (cell_int && ah > hextra) {
         if ((ah < adzx && hE < eps) || (ah < dzW && hW < eps)) qxq = 0.f;
@@ -373,18 +363,46 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
     pdzx = s.dzx; pdzy = s.dzy;
     p0h = c0h; p0x = c0x; p0y = c0y; p1h = c1h; p1x = c1x; p1y = c1y;
   }
-  // wave min of the float dt (positive floats order like their bit patterns)
+  return best;
+}
+
+// Wave min of a float dt (positive floats order like their bit patterns);
+// lane 0 publishes it as the FP64 value for the member's close.
+__device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float best, int lane) {
   unsigned int bits = __float_as_uint(best);
   for (int off = 32; off > 0; off >>= 1) {
-    unsigned int o = __shfl_xor(bits, off, 64);
+    const unsigned int o = __shfl_xor(bits, off, 64);
     bits = o < bits ? o : bits;
   }
   if (lane == 0) {
-    (void)nblocks;
     const float m = __uint_as_float(bits);
-    const unsigned long long wb = m >= kMaxDtF * 2.f ? ~0ull : dbits(double(m));
-    close_step_cursor(P, cur, wb, true);
+    publish_min(cc, m >= kMaxDtF * 2.f ? ~0ull : dbits(double(m)));
   }
+}
+
+// Batched step: wave w (placed XCD-contiguously) walks its pieces in order
+// and publishes one dt min per run of pieces of the same member.
+__global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
+  const int w = xcd_slot(int(blockIdx.x), a.nwaves);
+  const int lane = threadIdx.x;
+  const int p0 = a.wave_first[w], p1 = a.wave_first[w + 1];
+  int mem = -1;
+  StepCtrl* cc = nullptr;
+  float acc = kMaxDtF * 2.f;
+  for (int pi = p0; pi < p1; pi++) {
+    const Piece pc = a.pieces[pi];
+    const InstF32& P = a.inst[pc.member];
+    const int cur = *P.cursor;
+    if (P.ctrl[cur].halt) continue;
+    if (pc.member != mem) {
+      if (cc) publish_min_f32(cc, acc, lane);
+      mem = pc.member;
+      cc = P.ctrl + cur;
+      acc = kMaxDtF * 2.f;
+    }
+    acc = fminf(acc, step_piece(P, cur, pc.bx, pc.row_first, pc.row_last, lane));
+  }
+  if (cc) publish_min_f32(cc, acc, lane);
 }
 
 // Initial dt of one member over every cell (ghosts included).
@@ -395,36 +413,31 @@ __global__ __launch_bounds__(64) void k_prime_f32(const InstF32* inst, int membe
   float best = kMaxDtF * 2.f;
   for (int id = blockIdx.x * kLanes + threadIdx.x; id < n; id += gridDim.x * kLanes)
     best = fminf(best, cell_dt32(P.h[cur][id], P.qx[cur][id], P.qy[cur][id], P.cn_dx_f, P.hextra_f));
-  unsigned int bits = __float_as_uint(best);
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned int o = __shfl_xor(bits, off, 64);
-    bits = o < bits ? o : bits;
-  }
-  if (threadIdx.x == 0) {
-    const float m = __uint_as_float(bits);
-    const unsigned long long wb = m >= kMaxDtF * 2.f ? ~0ull : dbits(double(m));
-    StepCtrl* cc = P.ctrl + cur;
-    if (wb != ~0ull) atomicMin(&cc->dtmin_bits, wb);
-    __threadfence();
-    unsigned prev = atomicAdd(&cc->arrivals, 1u);
-    if (prev == gridDim.x - 1) {
-      __threadfence();
-      unsigned long long mm = atomicMin(&cc->dtmin_bits, ~0ull);
-      prepare_next(P, *cc, P.ctrl[cur ^ 1], mm, false, cur ^ 1);
-      __threadfence();
-      *P.cursor = cur ^ 1;
-    }
-  }
+  publish_min_f32(P.ctrl + cur, best, int(threadIdx.x));
+}
+
+// One thread per member (members[i], or first + i when members is NULL):
+// prepare the other control slot and flip the member's cursor.  Halted
+// members keep their cursor, so launches queued past a halt are no-ops.
+__global__ __launch_bounds__(64) void k_close_f32(const InstF32* inst, const int* members, int first,
+                                                  int n, int after_step) {
+  const int i = int(blockIdx.x) * kLanes + int(threadIdx.x);
+  if (i >= n) return;
+  const InstF32& P = inst[members ? members[i] : first + i];
+  const int cur = *P.cursor;
+  if (P.ctrl[cur].halt) return;
+  close_one(P, cur, after_step != 0);
+  *P.cursor = cur ^ 1;
 }
 
 }  // namespace
 
 int fused_grid_x_f32(int cols) { return (cols - 2 + kOutCols - 1) / kOutCols; }
 
-// Rows per wave for a grid stepped on its own: long strips amortise the
-// two-row halo and the per-wave set-up (measured best: 96 rows on a 4096^2
-// grid, about 3 waves per SIMD), short ones give a small grid enough waves
-// (16 rows on 1024^2).
+// Rows per wave of the FP64 fused step: long strips amortise the two-row
+// halo and the per-wave set-up (measured best: 96 rows on a 4096^2 grid,
+// about 3 waves per SIMD), short ones give a small grid enough waves (16 rows
+// on 1024^2).
 int default_strip_rows(int rows, int cols) {
   const int nbx = fused_grid_x_f32(cols);
   const int64_t target_waves = 3072;
@@ -432,9 +445,11 @@ int default_strip_rows(int rows, int cols) {
   return strip < 16 ? 16 : (strip > 128 ? 128 : int(strip));
 }
 
-int launch_step_f32(const BatchArgs& a, int nwork, hipStream_t s) {
-  if (nwork <= 0) return TR_OK;
-  hipLaunchKernelGGL(k_step_f32, dim3(unsigned(nwork)), dim3(kLanes), 0, s, a);
+int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s) {
+  if (nwaves <= 0) return TR_OK;
+  hipLaunchKernelGGL(k_step_f32, dim3(unsigned(nwaves)), dim3(kLanes), 0, s, a);
+  hipLaunchKernelGGL(k_close_f32, dim3(unsigned((a.nmembers + kLanes - 1) / kLanes)), dim3(kLanes), 0, s,
+                     a.inst, a.members, 0, a.nmembers, 1);
   return launch_check("step_f32");
 }
 
@@ -442,6 +457,7 @@ int launch_prime_f32(const InstF32* d_inst, int member, const InstF32& host_inst
   int64_t n = int64_t(host_inst.rows) * host_inst.cols;
   unsigned g = unsigned(std::min<int64_t>((n + kLanes - 1) / kLanes, 2048));
   hipLaunchKernelGGL(k_prime_f32, dim3(g), dim3(kLanes), 0, s, d_inst, member);
+  hipLaunchKernelGGL(k_close_f32, dim3(1), dim3(kLanes), 0, s, d_inst, nullptr, member, 1, 0);
   return launch_check("prime_f32");
 }
 

@@ -25,8 +25,8 @@ namespace tr {
 constexpr int kLanes = 64;
 constexpr int kOutCols = 60;
 
-// Initial dt: per-cell CFL over every cell (ghosts included), then the last
-// wave prepares the control block of step 0.
+// Initial dt: per-cell CFL over every cell (ghosts included); the close
+// kernel then prepares the control block of step 0.
 __global__ __launch_bounds__(64) void k_prime_f64(StepParams<double> p) {
   const int n = p.rows * p.cols;
   unsigned long long best = ~0ull;
@@ -37,23 +37,29 @@ __global__ __launch_bounds__(64) void k_prime_f64(StepParams<double> p) {
     best = b < best ? b : best;
   }
   best = wave_min_bits(best);
-  if (threadIdx.x == 0) close_step(p, best, false, gridDim.x);
+  if (threadIdx.x == 0) publish_min(p.ctrl + p.cur, best);
+}
+
+// One thread: prepare the next slot from this step's block, or carry a
+// halted block so launches queued past the halt stay no-ops.
+__global__ __launch_bounds__(64) void k_close_f64(StepParams<double> p, int after_step) {
+  if (threadIdx.x != 0) return;
+  if (after_step && p.ctrl[p.cur].halt) carry_halted(p);
+  else close_one(p, p.cur, after_step != 0);
 }
 
 template <bool kOpen>
 __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const int lane = threadIdx.x;
   const StepCtrl* cc = p.ctrl + p.cur;
-  if (cc->halt) {
-    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) carry_halted(p);
-    return;
-  }
+  if (cc->halt) return;
   const double dt = cc->dt;
   const int roff_row = cc->roff_row;
   const double* __restrict__ src_add = p.src_add + p.cur * p.nsrc;
   const double dx = p.dx, hextra = p.hextra;
   const int R = p.rows, C = p.cols;
-  const int bx = blockIdx.x, by = blockIdx.y;
+  const int tile = xcd_slot(int(blockIdx.x), int(p.nblocks));
+  const int bx = tile % p.nbx, by = tile / p.nbx;
 
   const int c = kOutCols * bx - 1 + lane;
   const bool col_ok = c >= 0 && c < C;
@@ -102,8 +108,8 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     // B: x face (c, c+1) of row r
     double x0h = 0.0, x0x = 0.0, x0y = 0.0, x1h, x1x, x1y;
     {
-      const double hR = __shfl_down(sh, 1), qxR = __shfl_down(sqx, 1), qyR = __shfl_down(sqy, 1);
-      const double zR = __shfl_down(sz, 1), sqR = __shfl_down(ssq, 1);
+      const double hR = from_right(sh), qxR = from_right(sqx), qyR = from_right(sqy);
+      const double zR = from_right(sz), sqR = from_right(ssq);
       double e0 = 0.0, e1 = 0.0, e2 = 0.0;
       if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
         Face64 f;
@@ -116,9 +122,9 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
           e2 = dt * f.d2[2] / dx;
         }
       }
-      x1h = __shfl_up(e0, 1);
-      x1x = __shfl_up(e1, 1);
-      x1y = __shfl_up(e2, 1);
+      x1h = from_left(e0);
+      x1x = from_left(e1);
+      x1y = from_left(e2);
     }
 
     // C: y face between row r (south) and row r-1 (north)
@@ -160,8 +166,8 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     }
     if (kOpen) {
       // transmissive ghosts: west/east columns first, then north/south rows
-      const double rh = __shfl_down(nh, 1), rqx = __shfl_down(nqx, 1), rqy = __shfl_down(nqy, 1);
-      const double lh = __shfl_up(nh, 1), lqx = __shfl_up(nqx, 1), lqy = __shfl_up(nqy, 1);
+      const double rh = from_right(nh), rqx = from_right(nqx), rqy = from_right(nqy);
+      const double lh = from_left(nh), lqx = from_left(nqx), lqy = from_left(nqy);
       if (ru_int && c == 0) { nh = rh; nqx = rqx; nqy = rqy; }
       if (ru_int && c == C - 1) { nh = lh; nqx = lqx; nqy = lqy; }
       if (ru == 1) { ah = nh; aqx = nqx; aqy = nqy; }          // row 0 <- row 1
@@ -172,8 +178,8 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     const int e = r - 2;
     if (e >= e_min && e <= e_max && e >= 0) {
       double hq = ah, qxq = aqx, qyq = aqy;
-      const double hE = __shfl_down(ah, 1), zE = __shfl_down(az, 1);
-      const double hW = __shfl_up(ah, 1), zW = __shfl_up(az, 1);
+      const double hE = from_right(ah), zE = from_right(az);
+      const double hW = from_left(ah), zW = from_left(az);
       const bool cell_int = e >= 1 && e <= R - 2 && col_int;
       if (cell_int && hq > hextra) {
         const double lev = hq + az;
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     py0h = y0h; py0x = y0x; py0y = y0y;
   }
   best = wave_min_bits(best);
-  if (lane == 0) close_step(p, best, true, p.nblocks);
+  if (lane == 0) publish_min(p.ctrl + p.cur, best);
 }
 
 // Host launchers (called by the simulation driver).
@@ -208,13 +214,15 @@ int launch_prime_f64(const StepParams<double>& p, hipStream_t s) {
   int64_t n = int64_t(p.rows) * p.cols;
   unsigned g = unsigned(std::min<int64_t>((n + kLanes - 1) / kLanes, 2048));
   hipLaunchKernelGGL(k_prime_f64, dim3(g), dim3(kLanes), 0, s, p);
+  hipLaunchKernelGGL(k_close_f64, dim3(1), dim3(kLanes), 0, s, p, 0);
   return launch_check("prime_f64");
 }
 
 int launch_step_f64(const StepParams<double>& p, hipStream_t s) {
-  dim3 grid(unsigned(p.nbx), unsigned(p.nby));
+  dim3 grid(p.nblocks);
   if (p.open_bc) hipLaunchKernelGGL(k_step_f64<true>, grid, dim3(kLanes), 0, s, p);
   else hipLaunchKernelGGL(k_step_f64<false>, grid, dim3(kLanes), 0, s, p);
+  hipLaunchKernelGGL(k_close_f64, dim3(1), dim3(kLanes), 0, s, p, 1);
   return launch_check("step_f64");
 }
 
