@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark of the fused timestep on one MI355X.
+"""Benchmark of the TRITON timestep on one MI355X.
 
-One "step" is one fused pass of the timestep over the workload's grid (or,
-for the ensemble, over every member).  Prints one JSON line with GCUPS
-(interior cells x steps / s / 1e9), the HBM roofline of the step kernel,
-and the CPU oracle timed on the host cores on a bounded sample.
+Default workload: the 64-member ensemble (square fBm watersheds of sides
+log-uniform in 256..2048, seeds 4000+i, Manning scale U(0.7,1.3), rain peak
+U(20,80) mm/h, 2 h horizon) stepped in FP32 by the batched kernel, one launch
+per step for all members, each with its own adaptive dt.  One "step" is one
+batched launch.  The same members are then stepped one at a time by the plain
+FP64 fused form for the FP32-batched vs FP64-plain ratio.
 
-  python bench.py [--steps K] [--warmup W] [--workload thacker|circular]
-                  [--form f64] [--cpu-seconds S]
+Prints one JSON line: GCUPS (interior cells x steps / s / 1e9, summed over
+members), the HBM roofline of the step kernel (algorithmic bytes per cell x
+cells per launch / measured launch time), the HBM traffic per launch from the
+rocprofv3 counter summary when one is committed for the workload, and the CPU
+oracle timed on the host cores on a bounded sample.
+
+  python bench.py [--steps K] [--warmup W] [--workload ensemble|watershed|thacker|circular]
+                  [--form f32|f64] [--cpu-seconds S] [--no-fp64]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -23,120 +30,269 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "GCUPS & % of HBM roofline, FP32-batched vs FP64-plain; serving p50/p99 latency"
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, AMD specification
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="thacker", choices=["thacker", "circular"])
-    ap.add_argument("--form", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="ensemble", choices=["ensemble", "watershed", "thacker", "circular"])
+    ap.add_argument("--form", default=None, choices=["f32", "f64"],
+                    help="single-grid workloads: fused form (default f32; the ensemble is always f32)")
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="timed steps per member of the FP64-plain comparison (default: --steps, after the "
+                         "same --warmup, so both forms time the same step window; 0: skip)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-plain comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU work for the oracle baseline (0: skip)")
     return ap.parse_args()
 
 
-def build_workload(name, form):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# Algorithmic bytes per interior cell-update of one fused pass: state read and
+# written, plus the spatially varying static fields the step must read.
+#   FP32: h,qx,qy read+write 24 B (+ per-face bed differences dzx,dzy 8 B on a
+#         non-flat bed) (+ Manning n^2 4 B when n varies per cell)
+#   FP64: h,qx,qy read+write 48 B (+ bed 8 B) (+ n^2 8 B)
+def b_alg(f32: bool, bed: bool, friction: bool) -> int:
+    if f32:
+        return 24 + (8 if bed else 0) + (4 if friction else 0)
+    return 48 + (8 if bed else 0) + (8 if friction else 0)
+
+
+def interior(case) -> int:
+    return (case.rows - 2) * (case.cols - 2)
+
+
+def cpu_baseline(cases, target_s):
+    """The FP64 oracle (the reference's unfused kernel sequence restated in C
+    with OpenMP) on the host cores: the first steps of the workload's grids,
+    taken in order, until about target_s seconds of work are done."""
+    from oracle.pyoracle import Oracle, num_threads
+    done_cells = 0
+    spent = 0.0
+    used = []
+    for case in cases:
+        if spent >= target_s:
+            break
+        cells = interior(case)
+        probe = Oracle.from_case(case)
+        t0 = time.perf_counter()
+        probe.run(max_steps=2)
+        per_step = max((time.perf_counter() - t0) / 2, 1e-6)
+        budget = target_s - spent if len(cases) == 1 else min(target_s - spent, target_s / 4)
+        nsteps = max(2, int(budget / per_step))
+        orc = Oracle.from_case(case)
+        t0 = time.perf_counter()
+        r = orc.run(max_steps=nsteps)
+        dt = time.perf_counter() - t0
+        spent += dt
+        done_cells += cells * int(r["steps"])
+        used.append(f"{case.rows - 2}x{case.cols - 2}:{int(r['steps'])}")
+    return {"value": round(done_cells / spent / 1e9, 4), "unit": "GCUPS", "cores": num_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle (C/OpenMP restatement of the reference's unfused kernels), first steps "
+                      f"from t=0 of grids [interior:steps] {', '.join(used)}; {spent:.1f} s"}
+
+
+def traffic_for(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    copy of known size), or None."""
+    try:
+        table = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    rec = table.get(f"{kernel}/{workload}")
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def events(stream):
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    return e0, e1
+
+
+def run_ensemble(args):
+    import torch
+    import triton_amd as ta
     from triton_amd import scenarios
-    f32 = form == "f32"
-    if name == "thacker":
+
+    t0 = time.perf_counter()
+    members = scenarios.ensemble()
+    log(f"generated {len(members)} members in {time.perf_counter() - t0:.1f} s")
+    cells = sum(interior(m) for m in members)
+    fric = True
+    batch = ta.Batch()
+    for m in members:
+        batch.add(m)
+    stream = torch.cuda.ExternalStream(batch.stream)
+    batch.run(max_steps=args.warmup)
+    batch.arm(max_steps=args.steps)
+    torch.cuda.synchronize()
+    e0, e1 = events(stream)
+    w0 = time.perf_counter()
+    e0.record(stream)
+    batch.enqueue(args.steps)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - w0
+    batch.sync()
+    stats = [batch.status(i) for i in sorted(batch.members)]
+    steps_taken = {s["steps"] for s in stats}
+    if steps_taken != {args.warmup + args.steps} or any(s["finished"] for s in stats):
+        log(f"a member finished inside the timed window (steps {sorted(steps_taken)}); lower --steps")
+        sys.exit(1)
+    ms = e0.elapsed_time(e1)
+    per_launch_s = ms / 1e3 / args.steps
+    gcups = cells * args.steps / (ms / 1e3) / 1e9
+    simt = [s["simtime"] for s in stats]
+    log(f"fp32 batched: {gcups:.2f} GCUPS, {ms / args.steps:.4f} ms per launch, waves {batch.work_waves}")
+    ba = b_alg(True, True, fric)
+    achieved = ba * cells / per_launch_s / 1e9
+    traffic = traffic_for("k_step_f32", "ensemble64")
+    res = {
+        "value": gcups, "ms_per_step": ms / args.steps, "wall": wall, "dtype": "fp32",
+        "config": {"workload": "ensemble64_fbm_watersheds_256-2048", "members": len(members),
+                   "interior_cells": cells, "sides": [m.rows - 2 for m in members], "horizon_h": 2.0,
+                   "form": "fp32_batched (FP32 state, FP64 time and volume)", "boundaries": "open",
+                   "waves_per_launch": batch.work_waves,
+                   "simtime_s_range_after": [round(min(simt), 1), round(max(simt), 1)]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "k_step_f32", "bytes_per_cell": ba, "cells_per_launch": cells},
+    }
+    batch.close()
+    fp64_steps = args.steps if args.fp64_steps is None else args.fp64_steps
+    if not args.no_fp64 and fp64_steps > 0:
+        res["fp64_plain"] = fp64_plain(members, args.warmup, fp64_steps)
+        res["speedup_fp32_batched_vs_fp64_plain"] = round(gcups / res["fp64_plain"]["value"], 3)
+    res["cases"] = members
+    return res
+
+
+def fp64_plain(members, warmup, steps):
+    """The same members stepped one at a time by the plain FP64 fused form
+    (one launch per step per member) over the same step window as the FP32
+    batch (after the same warmup), timed on each member's stream."""
+    import torch
+    import triton_amd as ta
+    total_ms = 0.0
+    total_cells = 0
+    for m in members:
+        sim = ta.Simulation(m, ta.FORM_F64_FUSED)
+        stream = torch.cuda.ExternalStream(sim.stream)
+        sim.run(max_steps=warmup)
+        sim.arm(max_steps=steps)
+        torch.cuda.synchronize()
+        e0, e1 = events(stream)
+        e0.record(stream)
+        sim.enqueue(steps)
+        e1.record(stream)
+        e1.synchronize()
+        st = sim.sync()
+        if st["steps"] != warmup + steps:
+            log("an FP64 member finished inside its timed window")
+            sys.exit(1)
+        total_ms += e0.elapsed_time(e1)
+        total_cells += interior(m) * steps
+        sim.close()
+    g = total_cells / (total_ms / 1e3) / 1e9
+    log(f"fp64 plain: {g:.2f} GCUPS over {len(members)} members x {steps} steps")
+    return {"value": round(g, 3), "unit": "GCUPS", "warmup": warmup, "steps_per_member": steps,
+            "form": "fp64_fused, members one at a time", "kernel": "k_step_f64",
+            "bytes_per_cell": b_alg(False, True, True)}
+
+
+def run_single(args):
+    import torch
+    import triton_amd as ta
+    from triton_amd import scenarios
+    f32 = (args.form or "f32") == "f32"
+    if args.workload == "watershed":
+        case = scenarios.watershed()
+        desc = {"workload": "watershed_4096x4096_dx10", "boundaries": "open", "rain_peak_mmh": 50.0,
+                "inflows": 4, "hours": 6.0}
+        fric, bed = True, True
+    elif args.workload == "thacker":
         case = scenarios.thacker()
-        desc = {"workload": "thacker_planar_1024x1024", "cells": 1024 * 1024, "dx_m": 8000.0 / 1024,
-                "h0_m": 10.0, "a_m": 3000.0, "eta_m": 0.5, "boundaries": "walls", "manning_n": 0.0}
-        # FP64: h, qx, qy read + written (48 B) + bed z (8 B);
-        # FP32: 24 B of state + two per-face bed differences (8 B)
-        b_alg = 32 if f32 else 56
+        desc = {"workload": "thacker_planar_1024x1024", "dx_m": 8000.0 / 1024, "h0_m": 10.0, "a_m": 3000.0,
+                "eta_m": 0.5, "boundaries": "walls", "manning_n": 0.0}
+        fric, bed = False, True
     else:
         case = scenarios.circular_dambreak()
-        desc = {"workload": "circular_dambreak_400x400", "cells": 400 * 400, "dx_m": 1.0,
-                "boundaries": "walls", "manning_n": 0.0}
-        b_alg = 24 if f32 else 48
-    return case, desc, b_alg
-
-
-def cpu_baseline(case, target_s):
-    """The FP64 oracle (the reference's unfused kernel sequence restated in C
-    with OpenMP) on the host cores, on the first steps of the same workload."""
-    from oracle.pyoracle import Oracle, num_threads
-    cells = (case.rows - 2) * (case.cols - 2)
-    probe = Oracle.from_case(case)
-    t0 = time.perf_counter()
-    probe.run(max_steps=5)
-    per_step = max((time.perf_counter() - t0) / 5, 1e-6)
-    nsteps = max(5, int(target_s / per_step))
-    orc = Oracle.from_case(case)
-    t0 = time.perf_counter()
-    r = orc.run(max_steps=nsteps)
-    dt = time.perf_counter() - t0
-    steps = int(r["steps"])
-    return {"value": round(cells * steps / dt / 1e9, 4), "unit": "GCUPS", "cores": num_threads(),
-            "kind": "port",
-            "sample": f"first {steps} steps of the same workload from t=0, FP64 oracle "
-                      f"(C/OpenMP restatement of the unfused reference kernels), {dt:.1f} s"}
+        desc = {"workload": "circular_dambreak_400x400", "dx_m": 1.0, "boundaries": "walls", "manning_n": 0.0}
+        fric, bed = False, False
+    cells = interior(case)
+    sim = ta.Simulation(case, ta.FORM_F32_FUSED if f32 else ta.FORM_F64_FUSED)
+    stream = torch.cuda.ExternalStream(sim.stream)
+    sim.run(max_steps=args.warmup)
+    st0 = sim.sync()
+    sim.arm(max_steps=args.steps)
+    torch.cuda.synchronize()
+    e0, e1 = events(stream)
+    w0 = time.perf_counter()
+    e0.record(stream)
+    sim.enqueue(args.steps)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - w0
+    st1 = sim.sync()
+    if st1["steps"] - st0["steps"] != args.steps:
+        log("the run ended inside the timed window; lower --steps")
+        sys.exit(1)
+    ms = e0.elapsed_time(e1)
+    gcups = cells * args.steps / (ms / 1e3) / 1e9
+    ba = b_alg(f32, bed, fric)
+    achieved = ba * cells / (ms / 1e3 / args.steps) / 1e9
+    kern = "k_step_f32" if f32 else "k_step_f64"
+    sim.close()
+    return {
+        "value": gcups, "ms_per_step": ms / args.steps, "wall": wall, "dtype": "fp32" if f32 else "fp64",
+        "config": dict(desc, interior_cells=cells, form="fp32_fused" if f32 else "fp64_fused",
+                       simtime_s=[round(st0["simtime"], 3), round(st1["simtime"], 3)]),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_for(kern, desc["workload"]), "kernel": kern, "bytes_per_cell": ba,
+                     "cells_per_launch": cells},
+        "cases": [case],
+    }
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this build "
-              "runs a single MI355X; sharding across GPUs is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): every run of this build "
+              "uses a single MI355X; how the path shards over GPUs is described in DESIGN.md",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import triton_amd as ta
-
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
-    case, desc, b_alg = build_workload(args.workload, args.form)
-    cells = (case.rows - 2) * (case.cols - 2)
-    form = ta.FORM_F32_FUSED if args.form == "f32" else ta.FORM_F64_FUSED
-    sim = ta.Simulation(case, form)
-    stream = torch.cuda.ExternalStream(sim.stream)
-
-    # warmup, then check that the timed steps stay inside the run
-    sim.run(max_steps=args.warmup)
-    st0 = sim.sync()
-    sim.arm(max_steps=args.steps)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    w0 = time.perf_counter()
-    ev0.record(stream)
-    sim.enqueue(args.steps)
-    ev1.record(stream)
-    ev1.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    st1 = sim.sync()
-    taken = st1["steps"] - st0["steps"]
-    if taken != args.steps:
-        print(f"only {taken} of {args.steps} timed steps ran (end of simulation reached); "
-              "lower --steps", file=sys.stderr)
-        sys.exit(1)
-    ms = ev0.elapsed_time(ev1)
-    sec = ms / 1e3
-    gcups = cells * args.steps / sec / 1e9
-    per_launch_s = sec / args.steps
-    achieved = b_alg * cells / per_launch_s / 1e9
-    cpu = None
-    if args.cpu_seconds > 0:
-        cpu = cpu_baseline(case, args.cpu_seconds)
+    ta.lib()  # fails loudly when the HIP library is missing
+    res = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
+    cases = res.pop("cases")
+    cpu = cpu_baseline(cases, args.cpu_seconds) if args.cpu_seconds > 0 else None
     out = {
-        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 5),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.form,
-        "data": "synthetic", "form": f"{'fp32' if args.form == 'f32' else 'fp64'}_fused",
-        "config": dict(desc, simtime_s=[round(st0["simtime"], 3), round(st1["simtime"], 3)]),
-        "wall_s": round(wall, 4),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "k_step_f32" if args.form == "f32" else "k_step_f64",
-                     "bytes_per_cell": b_alg},
+        "metric": METRIC, "value": round(res["value"], 3), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["ms_per_step"], 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res["dtype"],
+        "data": "synthetic (seeded fBm terrain, rain and inflow generators)",
+        "config": res["config"], "wall_s": round(res["wall"], 4), "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
-    sim.close()
-    print(json.dumps(out))
+    for k in ("fp64_plain", "speedup_fp32_batched_vs_fp64_plain"):
+        if k in res:
+            out[k] = res[k]
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
