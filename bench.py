@@ -11,11 +11,13 @@ FP64 fused form for the FP32-batched vs FP64-plain ratio.
 Prints one JSON line: GCUPS (interior cells x steps / s / 1e9, summed over
 members), the HBM roofline of the step kernel (algorithmic bytes per cell x
 cells per launch / measured launch time), the HBM traffic per launch from the
-rocprofv3 counter summary when one is committed for the workload, and the CPU
-oracle timed on the host cores on a bounded sample.
+rocprofv3 counter summary when one is committed for the workload, the CPU
+oracle timed on the host cores on a bounded sample, and (ensemble workload) a
+serving leg: Poisson arrivals of watershed requests under an HBM budget with
+continuous batching, reporting p50/p99 completion latency and GCUPS.
 
   python bench.py [--steps K] [--warmup W] [--workload ensemble|watershed|thacker|circular]
-                  [--form f32|f64] [--cpu-seconds S] [--no-fp64]
+                  [--form f32|f64] [--cpu-seconds S] [--no-fp64] [--serve-requests N]
 """
 from __future__ import annotations
 
@@ -47,6 +49,11 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-plain comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU work for the oracle baseline (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving leg (Poisson arrivals; 0: skip)")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="offered load of the serving leg against the measured batched GCUPS")
+    ap.add_argument("--serve-budget-gb", type=float, default=8.0, help="HBM budget of the serving leg")
     return ap.parse_args()
 
 
@@ -210,6 +217,44 @@ def fp64_plain(members, warmup, steps):
             "bytes_per_cell": b_alg(False, True, True)}
 
 
+def serving(args, capacity_gcups):
+    """Serving leg: Poisson arrivals (seed 5) of watershed requests, sides
+    round(2^U(8,12)), horizons U(1,6) h, request seeds 5000+k, rain as in the
+    watershed generator; continuous batching under an HBM budget.  The rate
+    is set for the requested offered load from an a-priori work estimate of
+    each request (interior cells x horizon / 1.2 s, the CFL dt of 1 m deep
+    water moving at 1 m/s on the 10 m grid)."""
+    import numpy as np
+    from triton_amd import scenarios
+    from triton_amd.serve import serve_poisson
+    rng = np.random.default_rng(5)
+    n = args.serve_requests
+    sides = [int(round(2.0 ** rng.uniform(8.0, 12.0))) for _ in range(n)]
+    hours = [float(rng.uniform(1.0, 6.0)) for _ in range(n)]
+    t0 = time.perf_counter()
+    cases = [scenarios.watershed(n=sd, seed=5000 + k, hours=hr, inflow_peaks=())
+             for k, (sd, hr) in enumerate(zip(sides, hours))]
+    log(f"generated {n} requests in {time.perf_counter() - t0:.1f} s")
+    est = np.array([sd * sd * hr * 3600.0 / 1.2 for sd, hr in zip(sides, hours)])
+    rate = args.serve_load * capacity_gcups * 1e9 / est.mean()
+    budget = int(args.serve_budget_gb * 2 ** 30)
+    t0 = time.perf_counter()
+    srv, offsets = serve_poisson(cases, rate, seed=5, hbm_budget_bytes=budget, chunk_steps=32)
+    wall = time.perf_counter() - t0
+    st = srv.stats
+    out = {"requests": n, "arrival_rate_per_s": round(rate, 4), "offered_load": args.serve_load,
+           "hbm_budget_gb": args.serve_budget_gb, "p50_latency_s": round(st.percentile(50), 3),
+           "p99_latency_s": round(st.percentile(99), 3), "max_latency_s": round(max(st.latencies), 3),
+           "gcups_wall": round(st.cell_steps / wall / 1e9, 3),
+           "gcups_busy": round(st.cell_steps / max(st.busy_s, 1e-9) / 1e9, 3),
+           "wall_s": round(wall, 2), "max_concurrent": st.max_members,
+           "peak_gb": round(st.peak_bytes / 2 ** 30, 3), "rounds": st.rounds, "chunk_steps": 32,
+           "sides_range": [min(sides), max(sides)], "horizon_h_range": [round(min(hours), 2), round(max(hours), 2)]}
+    srv.close()
+    log(f"serving: p50 {out['p50_latency_s']} s, p99 {out['p99_latency_s']} s, {out['gcups_wall']} GCUPS")
+    return out
+
+
 def run_single(args):
     import torch
     import triton_amd as ta
@@ -292,6 +337,8 @@ def main():
     for k in ("fp64_plain", "speedup_fp32_batched_vs_fp64_plain"):
         if k in res:
             out[k] = res[k]
+    if args.workload == "ensemble" and args.serve_requests > 0:
+        out["serving"] = serving(args, res["value"])
     print(json.dumps(out), flush=True)
 
 

@@ -231,6 +231,9 @@ int tr_batch_sync(tr_batch *b, int32_t *n_active);
 /* arm + enqueue/sync until no member is active or max_steps were queued. */
 int tr_batch_run(tr_batch *b, double t_stop, int64_t max_steps, int32_t *n_active);
 int tr_batch_status(tr_batch *b, int32_t id, tr_sim_status *st);
+/* Status as of the last tr_batch_sync / tr_batch_arm from the host mirror of
+ * the control blocks (no device work; volume is NaN). */
+int tr_batch_poll(tr_batch *b, int32_t id, tr_sim_status *st);
 int tr_batch_download(tr_batch *b, int32_t id, double *h, double *qx, double *qy, double *maxh);
 int64_t tr_batch_device_bytes(const tr_batch *b);
 int64_t tr_batch_member_cells(tr_batch *b, int32_t id);

@@ -1,0 +1,66 @@
+"""Serving loop on the device: requests admitted while others run give the
+same result as each request run alone (bit for bit: a member's arithmetic
+does not depend on the other members of its launches), and admission keeps
+the held device bytes within the HBM budget."""
+import numpy as np
+import pytest
+
+import triton_amd as ta
+from triton_amd import scenarios
+from triton_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def requests():
+    return [
+        scenarios.circular_dambreak(n=96, radius=12.0, t_end=6.0),
+        scenarios.thacker(n=128, periods=0.04),
+        scenarios.watershed(n=80, seed=11, hours=0.15, rain_peak_mmh=2000.0, inflow_peaks=()),
+    ]
+
+
+def alone(case):
+    with Server(chunk_steps=16) as srv:
+        rid = srv.submit(case)
+        srv.drain()
+        return srv.result(rid)
+
+
+def test_results_alone_equal_results_in_the_mix(gpu):
+    cases = requests()
+    ref = [alone(c) for c in cases]
+    need = [ta.Batch.member_bytes(c) for c in cases]
+    budget = need[0] + need[1] + 1024  # the third request must wait for a slot
+    assert need[2] + need[0] <= budget or need[2] + need[1] <= budget
+    with Server(hbm_budget_bytes=budget, chunk_steps=5) as srv:
+        ids = [srv.submit(cases[0])]
+        srv.step()
+        srv.step()
+        ids.append(srv.submit(cases[1]))
+        srv.step()
+        ids.append(srv.submit(cases[2]))
+        srv.step()
+        assert len(srv.running) <= 2 and srv.waiting
+        srv.drain()
+        assert srv.stats.peak_bytes <= budget
+        assert srv.stats.max_members == 2
+        for rid, r0 in zip(ids, ref):
+            r = srv.result(rid)
+            assert r.steps == r0.steps and r.status["simtime"] == r0.status["simtime"]
+            for k in ("h", "qx", "qy"):
+                np.testing.assert_array_equal(r.state[k], r0.state[k], err_msg=f"request {rid} {k}")
+            assert r.status["finished"]
+        # the waiting request was admitted only after an earlier one finished
+        third = srv.result(ids[2])
+        assert third.admitted >= min(srv.result(ids[0]).done, srv.result(ids[1]).done)
+
+
+def test_mass_of_a_served_request(gpu):
+    """Closed-domain request: the FP64 volume of the final FP32 state equals
+    the initial volume to 1e-6 relative."""
+    c = scenarios.circular_dambreak(n=96, radius=12.0, t_end=6.0)
+    dx = c.scalars["dx"]
+    v0 = float(c.arrays["h"][1:-1, 1:-1].sum()) * dx * dx
+    r = alone(c)
+    assert abs(r.status["volume"] - v0) / v0 < 1e-6

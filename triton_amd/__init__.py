@@ -129,6 +129,7 @@ def lib() -> C.CDLL:
         "tr_batch_sync": (C.c_int, [vp, C.POINTER(i32)]),
         "tr_batch_run": (C.c_int, [vp, f64, i64, C.POINTER(i32)]),
         "tr_batch_status": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
+        "tr_batch_poll": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
         "tr_batch_download": (C.c_int, [vp, i32, _p_d, _p_d, _p_d, _p_d]),
         "tr_batch_device_bytes": (i64, [vp]),
         "tr_batch_member_cells": (i64, [vp, i32]),
@@ -478,6 +479,12 @@ class Batch:
     def status(self, mid: int) -> dict:
         st = SimStatus()
         check(lib().tr_batch_status(self._h, int(mid), C.byref(st)))
+        return st.as_dict()
+
+    def poll(self, mid: int) -> dict:
+        """Status as of the last sync/arm, without device work (volume NaN)."""
+        st = SimStatus()
+        check(lib().tr_batch_poll(self._h, int(mid), C.byref(st)))
         return st.as_dict()
 
     def state(self, mid: int) -> dict[str, np.ndarray]:

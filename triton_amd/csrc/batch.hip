@@ -716,6 +716,23 @@ int tr_batch_status(tr_batch* b, int32_t id, tr_sim_status* st) {
   return TR_OK;
 }
 
+// Status of a member as of the last tr_batch_sync / tr_batch_arm, from the
+// host mirror of the control blocks: no device work.  volume is NaN.
+int tr_batch_poll(tr_batch* b, int32_t id, tr_sim_status* st) {
+  Member* mm = member(b, id);
+  if (!mm || !st) return TR_ERR_INVALID;
+  const StepCtrl& c = cur_ctrl(b, id);
+  std::memset(st, 0, sizeof(*st));
+  st->simtime = c.simtime;
+  st->dt = c.dt;
+  st->steps = c.step;
+  st->print_id = c.print_id;
+  st->finished = c.finished;
+  st->volume = NAN;
+  st->volume_in = c.vol_in;
+  return TR_OK;
+}
+
 int tr_batch_download(tr_batch* b, int32_t id, double* h, double* qx, double* qy, double* maxh) {
   Member* mm = member(b, id);
   if (!mm) return TR_ERR_INVALID;
