@@ -13,16 +13,21 @@
  * Layout: row-major rows x cols arrays that include the one-cell ghost ring
  * (rows = nrows_dem + 2, cols = ncols_dem + 2); row 0 is the north edge.
  *
- * Pinning (how each function is checked, see tests/test_oracle_*.py):
+ * Pinning (how each function is checked; tests/test_oracle_fixtures.py and
+ * tests/test_oracle_invariants.py):
  *   orc_compute_dt_and_sqrt, orc_find_min_dt, orc_flux_x, orc_flux_y,
  *   orc_update_cells (no friction), orc_wet_dry, orc_run loop/dt logic:
- *       reference fixtures test/reference/{circular,paraboloid}, plus
- *       lake-at-rest, mirror-symmetry and mass-conservation invariants.
+ *       reference fixtures test/reference/{circular,paraboloid} (identical
+ *       6-decimal output), plus exact lake at rest over a rough bed with dry
+ *       islands, x/y mirror and transpose symmetry of a centred dam break,
+ *       volume conservation to 1e-13, and Thacker's analytic solution.
  *   orc_update_cells (friction), orc_update_runoff, orc_compute_flow,
- *   orc_compute_extbc_values (type 3): reference fixture test/reference/allatoona.
+ *   orc_compute_extbc_values (type 3): reference fixture test/reference/allatoona;
+ *       friction also by the exact lake at rest with n = 0.03.
  *   orc_open_bc_west_east / orc_open_bc_north_south: no reference fixture;
- *       pinned by a uniform-flow invariant (exact steady state under
- *       transmissive ghosts) and D4 symmetry of a centred dam break.
+ *       pinned by an exact uniform-flow steady state, the symmetries above
+ *       with open boundaries, and a y-uniform dam break that must stay
+ *       y-uniform while water leaves through the east edge.
  *   orc_compute_extbc_values types 0/1/2: parity unpinned (no fixture, no
  *       closed form beyond the defining formula).
  */
