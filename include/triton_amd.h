@@ -15,7 +15,8 @@
  * Above the per-kernel functions sit the objects the reference has no single
  * call for: a loaded case (the .cfg/DEM/roughness/hydrograph input formats),
  * a device-resident simulation (the simulate() loop, src/triton_simulate.h:
- * 12-159), a batch of ensemble members and a serving queue.
+ * 12-159) and a batch of ensemble members, which the serving loop of the
+ * Python package (triton_amd/serve.py) drives.
  *
  * Thread-compatibility: distinct objects may be driven from distinct host
  * threads; one object must not be used by two threads at once.
@@ -51,9 +52,10 @@ const char *tr_version(void);
 /* ---- loaded case (input formats) ----------------------------------------- */
 typedef struct tr_case tr_case;
 
-/* Plain view of a case; every pointer is host memory owned by the case (or,
- * when filled by the caller for tr_sim_create_view, by the caller).  Arrays
- * are padded rows x cols unless stated. */
+/* Plain view of a case; every pointer is host memory owned by the case
+ * (tr_case_view_get) or by the caller (a view filled in by the caller and
+ * passed to tr_sim_create / tr_batch_add, which copy what they need).
+ * Arrays are padded rows x cols unless stated. */
 typedef struct tr_case_view {
   int32_t rows, cols;            /* padded dims: nrows_dem + 2, ncols_dem + 2 */
   double dx;                     /* cell size (m) */
@@ -224,8 +226,12 @@ int tr_batch_remove(tr_batch *b, int32_t id);
 /* Limits for the steps queued next, for every member: t_stop (<= 0: each
  * member's sim_duration) and at most max_steps more steps (< 0: unlimited). */
 int tr_batch_arm(tr_batch *b, double t_stop, int64_t max_steps);
-/* Queue nsteps batched launches without waiting. */
+/* Queue nsteps batched steps without waiting (as captured hipGraphs of
+ * graph_steps steps when tr_batch_set_launch enabled them). */
 int tr_batch_enqueue(tr_batch *b, int64_t nsteps);
+/* Launch shape: waves per SIMD of the row partition (0: 3) and steps per
+ * captured hipGraph (0: eager launches).  Results do not depend on either. */
+int tr_batch_set_launch(tr_batch *b, int32_t waves_per_simd, int32_t graph_steps);
 /* Wait; members past their limits leave the launch.  n_active may be NULL. */
 int tr_batch_sync(tr_batch *b, int32_t *n_active);
 /* arm + enqueue/sync until no member is active or max_steps were queued. */

@@ -130,6 +130,7 @@ def lib() -> C.CDLL:
         "tr_batch_run": (C.c_int, [vp, f64, i64, C.POINTER(i32)]),
         "tr_batch_status": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
         "tr_batch_poll": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
+        "tr_batch_set_launch": (C.c_int, [vp, i32, i32]),
         "tr_batch_download": (C.c_int, [vp, i32, _p_d, _p_d, _p_d, _p_d]),
         "tr_batch_device_bytes": (i64, [vp]),
         "tr_batch_member_cells": (i64, [vp, i32]),
@@ -480,6 +481,19 @@ class Batch:
         st = SimStatus()
         check(lib().tr_batch_status(self._h, int(mid), C.byref(st)))
         return st.as_dict()
+
+    def set_launch(self, waves_per_simd: int = 0, graph_steps: int = 0) -> None:
+        """Launch shape: waves per SIMD of the row partition (0: default) and
+        steps per captured hipGraph (0: eager).  Results do not change."""
+        check(lib().tr_batch_set_launch(self._h, int(waves_per_simd), int(graph_steps)))
+        self.launch = (int(waves_per_simd), int(graph_steps))
+
+    def autotune(self, candidates=None, steps: int = 8, cache: bool = True) -> tuple:
+        """Time a few launch shapes on the members now in the batch (each
+        candidate runs `steps` real steps, so no work is wasted) and keep the
+        fastest; see triton_amd.tune."""
+        from .tune import autotune_batch
+        return autotune_batch(self, candidates=candidates, steps=steps, cache=cache)
 
     def poll(self, mid: int) -> dict:
         """Status as of the last sync/arm, without device work (volume NaN)."""

@@ -109,12 +109,18 @@ struct tr_batch {
   int simd_count = 1024;
   int waves_per_simd = kWavesPerSimd;
   bool table_dirty = true;
+  // hipGraph of graph_steps batched steps, valid for one partition
+  int graph_steps = 0;         // 0: eager launches only
+  hipGraphExec_t gexec = nullptr;
+  BatchArgs gargs{};
+  int gsteps_built = 0;
   double* d_part = nullptr;
   double* d_sum = nullptr;
   double* h_sum = nullptr;
 
   ~tr_batch() {
     if (stream) hipStreamSynchronize(stream);
+    if (gexec) hipGraphExecDestroy(gexec);
     for (auto& mm : m)
       for (void* p : mm.allocs) hipFree(p);
     hipFree(d_inst);
@@ -639,6 +645,49 @@ int tr_batch_arm(tr_batch* b, double t_stop, int64_t max_steps) {
   return TR_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+bool same_args(const BatchArgs& x, const BatchArgs& y) {
+  return x.inst == y.inst && x.pieces == y.pieces && x.wave_first == y.wave_first &&
+         x.nwaves == y.nwaves && x.members == y.members && x.nmembers == y.nmembers;
+}
+
+// Capture graph_steps batched steps (step + close kernels only: no memset
+// or copy nodes) for the current partition.  The kernels read the piece
+// table and descriptors from device memory, so the graph stays valid while
+// the table pointers and the wave and member counts stay the same.
+int build_graph(tr_batch* b, const BatchArgs& a) {
+  if (b->gexec) {
+    hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  TR_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = TR_OK;
+  for (int k = 0; k < b->graph_steps && !rc; k++) rc = launch_step_f32(a, a.nwaves, b->stream);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->gexec = nullptr;
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  b->gargs = a;
+  b->gsteps_built = b->graph_steps;
+  return TR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
   if (!b || nsteps < 0) {
     set_error("tr_batch_enqueue: bad arguments");
@@ -646,7 +695,30 @@ int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
   }
   if (b->table_dirty) RC(rebuild_work(b));
   BatchArgs a{b->d_inst, b->d_pieces, b->d_first, b->nwork, b->d_members, b->nmembers};
-  for (int64_t k = 0; k < nsteps; k++) RC(launch_step_f32(a, b->nwork, b->stream));
+  int64_t k = 0;
+  if (b->graph_steps > 0 && a.nwaves > 0 && nsteps >= b->graph_steps) {
+    if (!b->gexec || b->gsteps_built != b->graph_steps || !same_args(a, b->gargs)) RC(build_graph(b, a));
+    for (; k + b->graph_steps <= nsteps; k += b->graph_steps) TR_HIP(hipGraphLaunch(b->gexec, b->stream));
+  }
+  for (; k < nsteps; k++) RC(launch_step_f32(a, b->nwork, b->stream));
+  return TR_OK;
+}
+
+// Launch-shape knobs (the autotuner's): waves per SIMD of the partition and
+// steps per captured graph (0: eager launches).  Results do not depend on
+// either.
+int tr_batch_set_launch(tr_batch* b, int32_t waves_per_simd, int32_t graph_steps) {
+  if (!b || waves_per_simd < 0 || waves_per_simd > 16 || graph_steps < 0 || graph_steps > 4096) {
+    set_error("tr_batch_set_launch: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  TR_HIP(hipStreamSynchronize(b->stream));
+  const int w = waves_per_simd > 0 ? waves_per_simd : kWavesPerSimd;
+  if (w != b->waves_per_simd) {
+    b->waves_per_simd = w;
+    b->table_dirty = true;
+  }
+  b->graph_steps = graph_steps;
   return TR_OK;
 }
 

@@ -1,0 +1,87 @@
+"""Autotuner of the batched launch shape.
+
+The batched FP32 step has two knobs that change speed but not results:
+the number of waves per SIMD the row partition is cut into, and how many
+steps are captured per hipGraph.  autotune_batch() times each candidate on
+the batch's own members (real steps, nothing is thrown away), keeps the
+fastest and remembers it per (device, member count, total cells) bucket in a
+small JSON cache ($TR_TUNE_CACHE, default ~/.cache/triton_amd/tune.json).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from pathlib import Path
+
+DEFAULT_CANDIDATES = ((2, 0), (3, 0), (4, 0), (3, 16))
+
+
+def cache_path() -> Path:
+    p = os.environ.get("TR_TUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "triton_amd" / "tune.json"
+
+
+def _load() -> dict:
+    try:
+        return json.loads(cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: dict) -> None:
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        tmp = p.with_suffix(".tmp")
+        tmp.write_text(json.dumps(table, indent=1, sort_keys=True))
+        tmp.replace(p)
+    except OSError:
+        pass  # a read-only home only costs the re-tune
+
+
+def bucket(batch) -> str:
+    import torch
+    cells = sum((c.rows - 2) * (c.cols - 2) for c in batch.members.values())
+    dev = torch.cuda.get_device_name(torch.cuda.current_device()) if torch.cuda.is_available() else "none"
+    return f"{dev}|m{len(batch.members)}|c2^{round(math.log2(max(cells, 1)))}"
+
+
+def time_steps(batch, steps: int) -> float:
+    """Milliseconds per batched step over `steps` steps (after as many
+    untimed ones, which also build any graph)."""
+    import torch
+    st = torch.cuda.ExternalStream(batch.stream)
+    batch.arm(max_steps=2 * steps)
+    batch.enqueue(steps)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    batch.enqueue(steps)
+    e1.record(st)
+    e1.synchronize()
+    batch.sync()
+    return e0.elapsed_time(e1) / steps
+
+
+def autotune_batch(batch, candidates=None, steps: int = 8, cache: bool = True) -> tuple:
+    """Pick (waves_per_simd, graph_steps) for `batch`; returns the choice.
+    The members advance by 2 * steps * len(candidates) steps at most; the
+    caller arms the batch again afterwards."""
+    key = bucket(batch)
+    table = _load() if cache else {}
+    if key in table:
+        choice = tuple(table[key]["launch"])
+        batch.set_launch(*choice)
+        return choice
+    cands = list(candidates or DEFAULT_CANDIDATES)
+    results = []
+    for w, g in cands:
+        batch.set_launch(w, min(g, steps) if g else 0)
+        results.append((time_steps(batch, steps), (w, g)))
+    best = min(results)[1]
+    batch.set_launch(*best)
+    if cache:
+        table[key] = {"launch": list(best), "ms": {f"{w},{g}": round(t, 5) for t, (w, g) in results}}
+        _store(table)
+    return best
