@@ -9,16 +9,18 @@ Tolerances (stated here and asserted):
       differ by an ulp on some inputs, and thin sheet flow carries that
       difference along: max |dh| <= 5e-5 m, max |dq| <= 1e-4 m^2/s, mean
       |dh| <= 1e-8 m, relative mass error <= 1e-12.
-  FP32 fused (batched kernel): max |dh| <= max(1e-3 m, 2 x floor), where the
-      floor is how far the FP64 oracle itself moves when its inputs are
-      rounded to FP32 (at a moving shoreline a cell can be wet in one run and
-      dry in the other); mean |dh| <= 1e-4 m; relative mass error <= 1e-6;
-      wet/dry mismatches in under 1 % of cells (the shoreline band).
-      Exception, the Thacker bowl: 10 m of water oscillating with a 3 mm
-      surface tilt changes depth by ~3e-6 m per step while FP32 resolves
-      ~1e-6 m at 10 m, so every step's update is rounded to a few ulps; there
-      max |dh| <= 1e-2 m and mean |dh| <= 2e-3 m, and its depth error against
-      the analytic solution is reported beside the FP64 forms'.
+  FP32 fused (batched kernel): max |dh| <= 1e-3 m over cells wet in both
+      runs and more than 3 cells from a wet/dry edge of either run; mean |dh|
+      <= 1e-4 m over all cells; relative mass error <= 1e-6; wet/dry
+      mismatches in under 1 % of cells.  Inside the shoreline band a cell can
+      be wet in one run and dry in the other, so pointwise differences there
+      reach the depth of a thin shoreline cell; the band's max is reported
+      (and, with the max over all cells, bounded by max(1e-3, 2 x floor) where
+      the floor is how far the FP64 oracle itself moves when its inputs are
+      rounded to FP32, except on the Thacker bowl, whose moving shoreline
+      sweeps many cells: there the band max is bounded by 1e-2 m).  The
+      Thacker bowl's depth error against the analytic solution is reported
+      for every form.
 
 Set TR_ACCURACY_REPORT=<path> to write the table as JSON lines."""
 import json
@@ -26,6 +28,7 @@ import os
 
 import numpy as np
 import pytest
+from scipy.ndimage import binary_dilation
 
 from oracle.pyoracle import Oracle
 import triton_amd as ta
@@ -78,6 +81,12 @@ def errors(case, ref, got, hextra):
     wr, wg = ref["h"][1:-1, 1:-1] > hextra, got["h"][1:-1, 1:-1] > hextra
     out["wet_mismatch"] = int((wr != wg).sum())
     out["cells"] = int(wr.size)
+    edges = (wr ^ binary_dilation(wr)) | (wg ^ binary_dilation(wg)) | (wr != wg)
+    band = binary_dilation(edges, iterations=3)
+    inner = wr & wg & ~band
+    dh = np.abs(got["h"][1:-1, 1:-1] - ref["h"][1:-1, 1:-1])
+    out["max_dh_inner"] = float(dh[inner].max()) if inner.any() else 0.0
+    out["max_dh_shoreline"] = float(dh[band].max()) if band.any() else 0.0
     return out
 
 
@@ -114,10 +123,11 @@ def test_error_against_oracle(gpu, name, form):
         assert e["wet_mismatch"] == 0
     else:
         assert e["simtime"] == pytest.approx(e["oracle_simtime"], rel=1e-12)
+        assert e["max_dh_inner"] <= 1e-3, e
+        assert e["mean_dh"] <= 1e-4, e
         if name.startswith("thacker"):
-            assert e["max_dh"] <= 1e-2 and e["mean_dh"] <= 2e-3, e
+            assert e["max_dh"] <= 1e-2, e
         else:
             assert e["max_dh"] <= max(1e-3, 2.0 * e["fp32_input_floor_dh"]), e
-            assert e["mean_dh"] <= 1e-4, e
         assert e["rel_mass"] <= 1e-6, e
         assert e["wet_mismatch"] <= 0.01 * e["cells"], e

@@ -60,15 +60,17 @@ struct Side {
 //  * both wet-wet clip bounds share one reciprocal;
 //  * the Harten-Hyman entropy fix runs only in waves where some lane needs it;
 //  * the front side is the total fluctuation minus the back side;
-//  * no final flush (the threshold's job is the dry tests above).
+//  * no final flush (the threshold's job is the dry tests above);
+//  * a face with both sides dry (h = 0) needs no mask: with the Roe-average
+//    reciprocal kept finite every wave strength, source and fluctuation of
+//    it is an exact zero.
 __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, float hdxdt,
                                        float eps, float& lm, float& ln, float& lt, float& rm,
                                        float& rn, float& rt) {
-  const bool active = L.h > 0.f || R.h > 0.f;
   const float g2 = (0.5f * kGf) * (L.h + R.h);
   const float ic = rsqf(g2);
   const float c = g2 * ic;
-  const float is = rcpf(L.sq + R.sq);
+  const float is = rcpf(fmaxf(L.sq + R.sq, 1e-30f));
   const float ub = (L.un * L.sq + R.un * R.sq) * is;
   const float vb = (L.ut * L.sq + R.ut * R.sq) * is;
   float l0 = ub - c, l2 = ub + c, e0 = 0.f, e2 = 0.f;
@@ -76,7 +78,7 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   const float aL2 = L.un + kSqrtGf * L.sq, aR2 = R.un + kSqrtGf * R.sq;
   const bool fix0 = aL0 < 0.f && aR0 > 0.f;
   const bool fix2 = aL2 < 0.f && aR2 > 0.f;
-  if (__any(active && (fix0 || fix2))) {
+  if (__any(fix0 || fix2)) {
     const float sp0 = aL0 * (aR0 - l0) * rcpf(aR0 - aL0);
     const float sp2 = aR2 * (l2 - aL2) * rcpf(aR2 - aL2);
     e0 = fix0 ? l0 - sp0 : 0.f;
@@ -96,7 +98,7 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   float b0 = (0.5f * kGf) * (hp0 - 0.5f * fabsf(dzp)) * dzp * ic;
   const float hp = L.h + a0;
   const bool sub = l0 * l2 < 0.f && hp > 0.f && L.h > 0.f && R.h > 0.f;
-  if (__any(active && sub)) {
+  if (__any(sub)) {
     const float d0 = l0 + e2, d2 = l2 + e0;
     const float rr = rcpf(d0 * d2);
     const float lo = l0 * d2 * rr * (a0 * l0 - L.h * hdxdt + e2 * R.h);
@@ -126,7 +128,7 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     tt = 0.f;
   }
   float fm = tm - bm, fn = tn - bn, ft = tt - bt;
-  if (__any(active && (fix0 || fix2))) {
+  if (__any(fix0 || fix2)) {
     const float q0 = e0 * a0, q2 = e2 * a2;
     const float r0 = e0 > 0.f ? q0 : 0.f, r2 = e2 > 0.f ? q2 : 0.f;
     const float k0 = q0 - r0, k2 = q2 - r2;
@@ -137,12 +139,12 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     fn += r0 * n0 + r2 * n2;
     ft += (r0 + r2) * vb;
   }
-  lm = active ? bm : 0.f;
-  ln = active ? bn : 0.f;
-  lt = active ? bt : 0.f;
-  rm = active ? fm : 0.f;
-  rn = active ? fn : 0.f;
-  rt = active ? ft : 0.f;
+  lm = bm;
+  ln = bn;
+  lt = bt;
+  rm = fm;
+  rn = fn;
+  rt = ft;
 }
 
 __device__ __forceinline__ float cell_dt32(float h, float qx, float qy, float cn_dx, float hextra) {
@@ -172,21 +174,33 @@ struct Raw {
   float h, qx, qy, dzx, dzy, n2, add;
 };
 
-// Row r, column c of the input, with r and c clamped into the grid: rows and
-// columns outside it only feed halo or ghost cells that are never updated,
-// so they may hold copies instead of zeros, and the loads need no branches.
-__device__ __forceinline__ Raw load_row(const Loc& L, int r, int c) {
+// Loads and stores by a 32-bit byte offset from a wave-uniform base: the
+// base stays in scalar registers and one vector add per row forms the offset
+// every array shares (members hold fewer than 2^30 cells).
+template <class T>
+__device__ __forceinline__ T ld(const TR_GLOBAL T* base, unsigned boff) {
+  return *(const TR_GLOBAL T*)((const TR_GLOBAL char*)base + boff);
+}
+template <class T>
+__device__ __forceinline__ TR_GLOBAL T& at(TR_GLOBAL T* base, unsigned boff) {
+  return *(TR_GLOBAL T*)((TR_GLOBAL char*)base + boff);
+}
+
+// Row r of the input at the lane's column (its byte offset col4 already
+// clamped into the grid), with r clamped too: rows and columns outside the
+// grid only feed halo or ghost cells that are never updated, so they may hold
+// copies instead of zeros, and the loads need no branches.
+__device__ __forceinline__ Raw load_row(const Loc& L, int r, unsigned col4) {
   const int rr = min(max(r, 0), L.rows - 1);
-  const int cc = min(max(c, 0), L.cols - 1);
-  const long long i = (long long)rr * L.cols + cc;
+  const unsigned b = unsigned(rr * L.cols) * 4u + col4;
   Raw w;
-  w.h = L.h[i];
-  w.qx = L.qx[i];
-  w.qy = L.qy[i];
-  w.dzx = L.dzx[i];
-  w.dzy = L.dzy[i];
-  w.n2 = L.n2 ? L.n2[i] : 0.f;
-  w.add = L.nz > 0 ? (L.rid ? float(L.zadd[L.rid[i]]) : L.add0) : 0.f;
+  w.h = ld(L.h, b);
+  w.qx = ld(L.qx, b);
+  w.qy = ld(L.qy, b);
+  w.dzx = ld(L.dzx, b);
+  w.dzy = ld(L.dzy, b);
+  w.n2 = L.n2 ? ld(L.n2, b) : 0.f;
+  w.add = L.nz > 0 ? (L.rid ? float(L.zadd[ld(L.rid, b)]) : L.add0) : 0.f;
   return w;
 }
 
@@ -228,11 +242,13 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   L.add0 = P.num_runoffs > 0 ? float(L.zadd[0]) : 0.f;
 
   const int c = kOutCols * bx - 1 + lane;
+  const unsigned col4 = unsigned(min(max(c, 0), Cc - 1)) * 4u;
   const bool col_int = c >= 1 && c <= Cc - 2;
   const bool out_lane = lane >= 2 && lane <= kOutCols + 1 && col_int;
   const bool ghost_w = c == 0 && lane == 1;
   const bool ghost_e = c == Cc - 1 && lane >= 3 && lane <= kLanes - 2;
   const bool col_mine = out_lane || ghost_w || ghost_e;
+  const bool edge_we = bx == 0 || bx == P.nbx - 1;
 
   // previous row: input state, derived values, and its RHS sums so far
   float ph = 0.f, pqx = 0.f, pqy = 0.f, pu = 0.f, pv = 0.f, psq = 0.f, pn2 = 0.f, padd = 0.f;
@@ -244,10 +260,13 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   float best = kMaxDtF * 2.f;
   int ks = 0;  // first source not above the row being updated
 
-  Raw nxt = load_row(L, e_min - 2, c);
-  for (int r = e_min - 2; r <= e_max + 2; ++r) {
+  // One code path for every row: a row's bits must not depend on where a
+  // piece starts (members give the same result alone and in any batch).
+  const int r_end = e_max + 2;
+  Raw nxt = load_row(L, e_min - 2, col4);
+  for (int r = e_min - 2; r <= r_end; ++r) {
     const Raw s = nxt;
-    if (r + 1 <= e_max + 2) nxt = load_row(L, r + 1, c);
+    if (r + 1 <= r_end) nxt = load_row(L, r + 1, col4);
     const float rs = rsqf(s.h);
     const float sq = s.h > 0.f ? s.h * rs : 0.f;
     const float ih = rs * rs;
@@ -326,10 +345,12 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
         if (src_col[k] == c) nh = nh + float(src_add[k]);
     }
     if (open_bc) {
-      const float rh = from_right(nh), rqx = from_right(nqx), rqy = from_right(nqy);
-      const float lh = from_left(nh), lqx = from_left(nqx), lqy = from_left(nqy);
-      if (ru_int && c == 0) { nh = rh; nqx = rqx; nqy = rqy; }
-      if (ru_int && c == Cc - 1) { nh = lh; nqx = lqx; nqy = lqy; }
+      if (edge_we) {  // this strip holds the west or east ghost column
+        const float rh = from_right(nh), rqx = from_right(nqx), rqy = from_right(nqy);
+        const float lh = from_left(nh), lqx = from_left(nqx), lqy = from_left(nqy);
+        if (ru_int && c == 0) { nh = rh; nqx = rqx; nqy = rqy; }
+        if (ru_int && c == Cc - 1) { nh = lh; nqx = lqx; nqy = lqy; }
+      }
       if (ru == 1) { ah = nh; aqx = nqx; aqy = nqy; }
       if (ru == R - 1) { nh = ah; nqx = aqx; nqy = aqy; }
     }
@@ -347,11 +368,11 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
         if ((ah < adzy && bh < eps) || (ah < -pdzy && nh < eps)) qyq = 0.f;
       }
       if (col_mine) {
-        const long long o = (long long)e * Cc + c;
-        ho[o] = ah;
-        qxo[o] = qxq;
-        qyo[o] = qyq;
-        if (maxh && cell_int && ah > maxh[o]) maxh[o] = ah;
+        const unsigned o = unsigned(e * Cc) * 4u + unsigned(c) * 4u;
+        at(ho, o) = ah;
+        at(qxo, o) = qxq;
+        at(qyo, o) = qyq;
+        if (maxh && cell_int && ah > at(maxh, o)) at(maxh, o) = ah;
         best = fminf(best, cell_dt32(ah, qxq, qyq, cn_dx, hextra));
       }
     }
@@ -392,7 +413,8 @@ __global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
   for (int pi = p0; pi < p1; pi++) {
     const Piece pc = a.pieces[pi];
     const InstF32& P = a.inst[pc.member];
-    const int cur = *P.cursor;
+    // wave-uniform: keeps the member's array bases in scalar registers
+    const int cur = __builtin_amdgcn_readfirstlane(*P.cursor);
     if (P.ctrl[cur].halt) continue;
     if (pc.member != mem) {
       if (cc) publish_min_f32(cc, acc, lane);
