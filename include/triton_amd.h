@@ -87,6 +87,11 @@ typedef struct tr_case_view {
 int tr_case_load(const char *cfg_path, const char *base_dir, tr_case **out);
 /* Fill `out` with pointers into the case (valid until tr_case_free). */
 int tr_case_view_get(tr_case *c, tr_case_view *out);
+/* Value of a .cfg key as the reference's parser reads it (first occurrence
+ * wins, '#' comments and quotes stripped, src/config_utils.h:195-238): copies
+ * at most buflen-1 bytes and a NUL into buf and returns the value's length;
+ * -1 when the key is absent. */
+int32_t tr_case_config_value(const tr_case *c, const char *key, char *buf, int32_t buflen);
 void tr_case_free(tr_case *c);
 
 /* Reference output writers (src/output_ascii.h:52-75, src/output_binary.h:65-80):
@@ -218,6 +223,11 @@ int tr_batch_create(int64_t hbm_budget_bytes, int32_t waves_per_simd, tr_batch *
 int64_t tr_batch_member_bytes(const tr_case_view *v);
 int tr_batch_add(tr_batch *b, const tr_case_view *v, int32_t *id);
 /* tune->dry_eps applies per member; the other tuning fields are the batch's. */
+/* With on != 0 the member stops after every step that completes a print
+ * interval (print_id advances) until the next tr_batch_arm, so its state can
+ * be written out at print times as the reference's loop does
+ * (src/triton_simulate.h:92-110). */
+int tr_batch_set_print_halt(tr_batch *b, int32_t id, int32_t on);
 int tr_batch_add_ex(tr_batch *b, const tr_case_view *v, const tr_tuning *tune, int32_t *id);
 /* Default rows per wave of the FP64 fused step for a padded rows x cols grid
  * (enough waves to fill the device; long strips otherwise). */

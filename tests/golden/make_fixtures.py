@@ -1,7 +1,9 @@
 """Copy the reference's own test data into tests/golden/ref/ (gzip-compressed).
 
 The files are data, not code: the .cfg files and input rasters of the three
-ctest cases (test/*.cfg, input/{circular,paraboloid,allatoona}/*) and the
+ctest cases (test/*.cfg, input/{circular,paraboloid,allatoona}/*), the
+ensemble job file and its two member configs (test/ensembles_ensify.yaml,
+test/paraboloid_ens{1,2}.cfg), and the
 FP64 outputs the reference's BIT4BIT tests compare against
 (test/reference/<case>/{H,QX,QY}_*.out).  Run in the build container, where
 the reference is mounted; the GPU host only reads the copies.
@@ -16,6 +18,7 @@ OUT = Path(__file__).resolve().parent / "ref"
 
 FILES = [
     "test/circular_dambreak.cfg", "test/paraboloid.cfg", "test/allatoona.cfg",
+    "test/ensembles_ensify.yaml", "test/paraboloid_ens1.cfg", "test/paraboloid_ens2.cfg",
     "test/reference/circular/H_01_00.out", "test/reference/circular/QX_01_00.out",
     "test/reference/circular/QY_01_00.out",
     "test/reference/paraboloid/H_12_00.out", "test/reference/paraboloid/QX_12_00.out",

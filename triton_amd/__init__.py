@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
         "tr_batch_run": (C.c_int, [vp, f64, i64, C.POINTER(i32)]),
         "tr_batch_status": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
         "tr_batch_poll": (C.c_int, [vp, i32, C.POINTER(SimStatus)]),
+        "tr_batch_set_print_halt": (C.c_int, [vp, i32, i32]),
+        "tr_case_config_value": (i32, [vp, C.c_char_p, C.c_char_p, i32]),
         "tr_batch_set_launch": (C.c_int, [vp, i32, i32]),
         "tr_batch_download": (C.c_int, [vp, i32, _p_d, _p_d, _p_d, _p_d]),
         "tr_batch_device_bytes": (i64, [vp]),
@@ -176,11 +178,14 @@ class Case:
     north (src/constants.h:82-112)."""
 
     ARRAYS_F = ("h", "qx", "qy", "dem", "nman2", "maxh")
+    # .cfg keys kept verbatim for the output writers
+    CONFIG_KEYS = ("output_folder", "output_format", "outfile_pattern", "print_option", "output_option")
 
     def __init__(self):
         self._handle = None
         self.arrays: dict[str, np.ndarray] = {}
         self.scalars: dict = {}
+        self.config: dict[str, str] = {}
 
     # -- construction -----------------------------------------------------
     @classmethod
@@ -218,6 +223,10 @@ class Case:
                   "init_dt", "time_series_flag", "checkpoint_id", "it_print"):
             self.scalars[k] = getattr(v, k)
         assert n == self.arrays["h"].size
+        buf = C.create_string_buffer(4096)
+        for k in cls.CONFIG_KEYS:
+            if L.tr_case_config_value(h, k.encode(), buf, len(buf)) >= 0:
+                self.config[k] = buf.value.decode()
         L.tr_case_free(h)
         self._handle = None
         return self
@@ -495,6 +504,11 @@ class Batch:
         from .tune import autotune_batch
         return autotune_batch(self, candidates=candidates, steps=steps, cache=cache)
 
+    def set_print_halt(self, mid: int, on: bool = True) -> None:
+        """Stop the member after each step that completes a print interval,
+        until the next arm(), so its state can be written at print times."""
+        check(lib().tr_batch_set_print_halt(self._h, int(mid), int(bool(on))))
+
     def poll(self, mid: int) -> dict:
         """Status as of the last sync/arm, without device work (volume NaN)."""
         st = SimStatus()
@@ -554,15 +568,34 @@ class Triton:
         return int(self.sim.sync()["print_id"])
 
     def write_output(self, output_dir, print_id: int, pattern="%s/%s/%s_%02d_%02d") -> None:
-        """ASC and BIN rasters of H, QX, QY (src/output.h:512-709, :737-752)."""
-        stt = self.sim.state()
-        L = lib()
-        for sub, fn in (("asc", L.tr_write_output_ascii), ("bin", L.tr_write_output_binary)):
-            os.makedirs(os.path.join(output_dir, sub), exist_ok=True)
-            for name, key in (("H", "h"), ("QX", "qx"), ("QY", "qy")):
-                path = (pattern % (str(output_dir), sub, name, print_id, 0)) + ".out"
-                a = np.ascontiguousarray(stt[key])
-                check(fn(path.encode(), a.shape[0], a.shape[1], _ptr(a, C.c_double)))
+        write_rasters(self.sim.state(), self.case, output_dir, print_id, pattern)
+
+
+def write_rasters(state: dict, case: Case, output_dir, print_id: int, pattern=None) -> list:
+    """Write a print's rasters as the reference's output::write_output does
+    (src/output.h:512-709): the .cfg's print_option picks H ('h'), QX ('u')
+    and QY ('v'); output_format ASC writes the ASCII raster and its BIN twin
+    (the hotstart input), BIN only the binary one; file names follow
+    outfile_pattern % (folder, format dir, variable, print id, rank 0)."""
+    cfg = getattr(case, "config", {}) or {}
+    pattern = pattern or cfg.get("outfile_pattern") or "%s/%s/%s_%02d_%02d"
+    opt = (cfg.get("print_option") or "huv").lower()
+    fmt = (cfg.get("output_format") or "ASC").upper()
+    L = lib()
+    writers = [("bin", L.tr_write_output_binary)]
+    if fmt != "BIN":
+        writers.append(("asc", L.tr_write_output_ascii))
+    written = []
+    for letter, name, key in (("h", "H", "h"), ("u", "QX", "qx"), ("v", "QY", "qy")):
+        if letter not in opt:
+            continue
+        a = np.ascontiguousarray(state[key])
+        for sub, fn in writers:
+            os.makedirs(os.path.join(str(output_dir), sub), exist_ok=True)
+            path = (pattern % (str(output_dir), sub, name, print_id, 0)) + ".out"
+            check(fn(path.encode(), a.shape[0], a.shape[1], _ptr(a, C.c_double)))
+            written.append(path)
+    return written
 
 
 def gpu_available() -> bool:

@@ -788,6 +788,16 @@ int tr_batch_status(tr_batch* b, int32_t id, tr_sim_status* st) {
   return TR_OK;
 }
 
+int tr_batch_set_print_halt(tr_batch* b, int32_t id, int32_t on) {
+  Member* mm = member(b, id);
+  if (!mm) return TR_ERR_INVALID;
+  TR_HIP(hipStreamSynchronize(b->stream));
+  mm->inst.halt_on_print = on ? 1 : 0;
+  RC(put_inst(b, id));
+  TR_HIP(hipStreamSynchronize(b->stream));
+  return TR_OK;
+}
+
 // Status of a member as of the last tr_batch_sync / tr_batch_arm, from the
 // host mirror of the control blocks: no device work.  volume is NaN.
 int tr_batch_poll(tr_batch* b, int32_t id, tr_sim_status* st) {

@@ -1,4 +1,5 @@
 // capi.cpp -- C ABI for loaded cases, writers and error reporting.
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -96,6 +97,22 @@ int tr_case_view_get(tr_case* cp, tr_case_view* v) {
   v->checkpoint_id = c.cfg.checkpoint_id;
   v->it_print = c.cfg.it_print;
   return TR_OK;
+}
+
+int32_t tr_case_config_value(const tr_case* c, const char* key, char* buf, int32_t buflen) {
+  if (!c || !key || buflen < 0 || (buflen > 0 && !buf)) {
+    tr::set_error("tr_case_config_value: bad arguments");
+    return -2;
+  }
+  auto it = c->c.cfg.raw.find(key);
+  if (it == c->c.cfg.raw.end()) return -1;
+  const std::string& v = it->second;
+  if (buflen > 0) {
+    const size_t n = std::min(v.size(), size_t(buflen - 1));
+    std::memcpy(buf, v.data(), n);
+    buf[n] = '\0';
+  }
+  return int32_t(v.size());
 }
 
 void tr_case_free(tr_case* c) { delete c; }
