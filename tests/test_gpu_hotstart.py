@@ -1,0 +1,45 @@
+"""Restart equivalence: stopping at a print, writing its BIN rasters and
+starting again from them (checkpoint_id, src/triton_init.h:303-353) gives
+the same steps and the same bits as one uninterrupted run."""
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+import triton_amd as ta
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("form", [ta.FORM_F64_FUSED, ta.FORM_F64_UNFUSED])
+def test_hotstart_equals_one_run(gpu, ref_tree, tmp_path, form):
+    cfg = ref_tree / "test" / "paraboloid.cfg"
+    case = ta.Case.load(cfg, ref_tree / "test")
+    with ta.Simulation(case, form) as sim:
+        mid = sim.run(t_stop=6 * case.scalars["print_interval"])
+        half = sim.state()
+        end = sim.run()
+        final = sim.state()
+    assert mid["print_id"] == 6 and end["print_id"] == 12
+
+    # the restart folder sits beside test/ so the cfg's ../input paths resolve
+    hot_dir = ref_tree / f"hot_{form}"
+    shutil.rmtree(hot_dir, ignore_errors=True)
+    hot_dir.mkdir()
+    text = cfg.read_text()
+    text = re.sub(r"(?m)^checkpoint_id=.*$", "checkpoint_id=6", text)
+    text = re.sub(r"(?m)^sim_start_time=.*$", f"sim_start_time={mid['simtime']!r}", text)
+    text = re.sub(r"(?m)^output_folder=.*$", f"output_folder={tmp_path / 'out'}", text)
+    (hot_dir / "hot.cfg").write_text(text)
+    ta.write_rasters(half, case, tmp_path / "out", 6)  # the print-6 rasters, as the run writes them
+    hot = ta.Case.load(hot_dir / "hot.cfg", hot_dir)    # reads out/bin/{H,QX,QY}_06_00.out
+    np.testing.assert_array_equal(hot.arrays["h"], half["h"])
+    assert hot.scalars["sim_start"] == mid["simtime"] and hot.scalars["checkpoint_id"] == 6
+    with ta.Simulation(hot, form) as sim:
+        st = sim.run()
+        again = sim.state()
+    assert st["print_id"] == 12 and st["simtime"] == end["simtime"]
+    assert st["steps"] == end["steps"] - mid["steps"]
+    for k in ("h", "qx", "qy"):
+        np.testing.assert_array_equal(again[k], final[k])
