@@ -120,3 +120,21 @@ def test_volume_balance_closed_domain(gpu):
     v_orc = orc.state["h"][1:-1, 1:-1].sum() * area
     assert abs(r["volume"] - v_orc) / v_orc < 1e-13
     assert abs(r["volume"] - v0) / v0 < 1e-6
+
+
+def test_allatoona_unfused_matches_reference(gpu, ref_tree):
+    """The reference's third fixture: Lake Allatoona with 156 runoff zones,
+    2 inflow hydrographs, 2 external boundary segments (Froude type) and
+    Manning friction, 4 h.  External boundaries run only in the unfused FP64
+    form, which must reproduce the reference's printed rasters at both
+    prints (the oracle does too, tests/test_oracle_fixtures.py)."""
+    case = ta.Case.load(ref_tree / "test/allatoona.cfg", base_dir=ref_tree / "test")
+    assert case.arrays["bc_index"].size > 0
+    with ta.Simulation(case, ta.FORM_F64_UNFUSED) as sim:
+        st = sim.run()
+        s = sim.state()
+    assert st["print_id"] == 2 and st["finished"]
+    for fld, k in (("H", "h"), ("QX", "qx"), ("QY", "qy")):
+        ref = load_asc(ref_tree / "test/reference/allatoona" / f"{fld}_02_00.out")
+        got = as_printed(s[k])
+        assert np.array_equal(got, ref), (fld, np.abs(got - ref).max(), int((got != ref).sum()))
