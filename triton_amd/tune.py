@@ -14,7 +14,7 @@ import math
 import os
 from pathlib import Path
 
-DEFAULT_CANDIDATES = ((2, 0), (3, 0), (4, 0), (3, 16))
+DEFAULT_CANDIDATES = ((3, 0), (4, 0), (4, 16))
 
 
 def cache_path() -> Path:
