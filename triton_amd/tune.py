@@ -14,7 +14,11 @@ import math
 import os
 from pathlib import Path
 
-DEFAULT_CANDIDATES = ((3, 0), (4, 0), (4, 16))
+# (waves per SIMD, steps per graph).  Up to 4 waves per SIMD are resident at
+# once (the step kernel's registers); more queue behind them, which evens out
+# uneven pieces (an ensemble of mixed sizes) but adds a tail when the pieces
+# are equal (one large grid).
+DEFAULT_CANDIDATES = ((4, 0), (6, 0), (8, 0))
 
 
 def cache_path() -> Path:
