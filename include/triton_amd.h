@@ -171,8 +171,8 @@ typedef struct tr_sim_status {
 /* Tuning of one simulation / batch member (zero fields take the defaults). */
 typedef struct tr_tuning {
   int32_t strip_rows;      /* FP64 fused form: rows per wave (0: by grid size) */
-  double dry_eps;          /* FP32 form: dry / flush threshold (0: 1e-10; the
-                              reference's FP32 build uses 1e-6,
+  double dry_eps;          /* FP32 form: dry / flush threshold (0: 1e-12, the FP64
+                              value; the reference's FP32 build uses 1e-6,
                               src/constants.h:43, which loses mass on
                               small-scale flows) */
   int32_t waves_per_simd;  /* FP32 form: waves per SIMD of the launch (0: 4) */

@@ -117,3 +117,46 @@ def test_f32_volume_closure(gpu):
     added = r["volume"] - v0
     assert r["volume_in"] > 0
     assert abs(added - r["volume_in"]) / r["volume_in"] < 1e-3
+
+
+SHAPES = [(1, 1), (1, 130), (130, 1), (2, 3), (60, 60), (61, 61), (64, 119)]
+
+
+@pytest.mark.parametrize("open_bc", [0, 1])
+def test_f32_degenerate_shapes_vs_oracle_and_batched(gpu, open_bc):
+    """Single-row, single-column and tiny grids: FP32 within 1e-3 m of the
+    oracle, finite everywhere, and each equal bit for bit alone and in a
+    batch with all the others (their pieces share launches)."""
+    cases = [synthetic(*s, seed=sum(s) + 7 * open_bc, open_bc=open_bc, dur=10.0) for s in SHAPES]
+    alone = []
+    for c in cases:
+        orc = Oracle.from_case(c)
+        orc.run()
+        with ta.Batch() as b:
+            mid = b.add(c)
+            b.run()
+            st, s = b.status(mid), b.state(mid)
+        assert st["finished"] and np.isfinite(s["h"]).all() and np.isfinite(s["qx"]).all()
+        assert np.abs(s["h"] - orc.state["h"]).max() <= MAX_DEPTH_ERR
+        alone.append((st, s))
+    with ta.Batch() as b:
+        ids = [b.add(c) for c in cases]
+        b.run()
+        for mid, (st_a, s_a) in zip(ids, alone):
+            assert b.status(mid)["steps"] == st_a["steps"]
+            s_b = b.state(mid)
+            for k in ("h", "qx", "qy"):
+                assert np.array_equal(s_a[k], s_b[k])
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (130, 1), (61, 61)])
+def test_unfused_degenerate_shapes_bitwise(gpu, shape):
+    case = synthetic(*shape, seed=sum(shape), open_bc=1, dur=10.0)
+    orc = Oracle.from_case(case)
+    ro = orc.run()
+    with ta.Simulation(case, ta.FORM_F64_UNFUSED) as sim:
+        rs = sim.run()
+        st = sim.state()
+    assert rs["steps"] == ro["steps"]
+    for k in ("h", "qx", "qy"):
+        assert np.array_equal(st[k], orc.state[k]), k

@@ -74,11 +74,11 @@ __global__ void k_sum_parts(const double* __restrict__ part, int n, double* out)
 
 constexpr int kVolBlocks = 256;
 
-// FP32 dry / flush threshold.  1e-6 (the reference's FP32 EPS12) flushes
-// fluxes of small-scale flows and loses mass (2e-4 relative on the 0.1 m deep
-// paraboloid fixture); 1e-12 lets FP32 round-off through as spurious flow;
-// 1e-10 kept the relative mass error at 5e-8 there.
-constexpr float kDefaultDryEpsF32 = 1e-10f;
+// FP32 dry / flush threshold: the reference's FP64 value, so thin films wet
+// and dry where the FP64 form's do.  1e-6 (the reference's FP32 EPS12)
+// flushes fluxes of small-scale flows and loses mass (2e-4 relative on the
+// 0.1 m deep paraboloid fixture).
+constexpr float kDefaultDryEpsF32 = 1e-12f;
 
 // Waves per SIMD of a batched launch (the step kernel's registers allow 4,
 // and 4 measured fastest with the band partition) and the band heights a

@@ -16,8 +16,8 @@
 // written in normal/tangential components so x and y faces share one routine.
 // Semantics follow the reference (src/kernels_flux.h, src/kernels_state.h,
 // src/kernels_wetdry.h, src/kernels.h:102-187) with these FP32 choices:
-//   * the dry / flush threshold is a member setting (default 1e-10) instead
-//     of 1e-12 (the reference's FP32 build uses 1e-6, src/constants.h:43);
+//   * the dry / flush threshold is a member setting (default 1e-12, the
+//     reference's FP64 value; its FP32 build uses 1e-6, src/constants.h:43);
 //   * bed tests z1+h1<z2 become h1<dz on the stored face difference;
 //   * the semi-implicit friction uses q = 2m/(1+sqrt(1+4tt)), the same value
 //     as -0.5(m - m sqrt(1+4tt))/tt without its cancellation for small tt.
@@ -37,6 +37,9 @@ constexpr int kOutCols = 60;
 constexpr float kGf = 9.81f;
 constexpr float kSqrtGf = 3.132091953f;
 constexpr float kMaxDtF = 1e10f;
+// 16 units of FP32 round-off (2^-20): above the few roundings that separate
+// the two halves of a balanced wave strength, far below any resolved flow.
+constexpr float kFlushRel = 9.5367431640625e-7f;
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 // 1/sqrt(x) for x >= 0; subnormal inputs (depths at wetting fronts) are
@@ -60,7 +63,9 @@ struct Side {
 //  * both wet-wet clip bounds share one reciprocal;
 //  * the Harten-Hyman entropy fix runs only in waves where some lane needs it;
 //  * the front side is the total fluctuation minus the back side;
-//  * no final flush (the threshold's job is the dry tests above);
+//  * the final flush of |d| < threshold is kept for the mass components, and
+//    on faces with a dry side a relative flush of the two wave strengths
+//    removes FP32 round-off from balanced (zero) fluctuations;
 //  * a face with both sides dry (h = 0) needs no mask: with the Roe-average
 //    reciprocal kept finite every wave strength, source and fluctuation of
 //    it is an exact zero.
@@ -106,7 +111,20 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     const float bc = fminf(fmaxf(b0, lo), hi);
     b0 = sub ? bc : b0;
   }
-  const float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
+  float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
+  // Still water against a higher dry bed is balanced: both strengths are
+  // exact zeros, and the reference's final flush (|d| < 1e-12,
+  // src/kernels_flux.h:255-264) removes their FP64 round-off.  FP32
+  // round-off is ~1e-7 of the terms and would wet the dry cell by ~1e-10 m
+  // per step, enough to defeat the wet/dry test; on faces with a dry side a
+  // strength within kFlushRel of the terms it is formed from is zeroed.
+  const bool dry = L.h < eps || R.h < eps;
+  if (__any(dry)) {
+    const bool z0 = fabsf(W0) <= kFlushRel * (fabsf(l0 * a0) + fabsf(b0));
+    const bool z2 = fabsf(W2) <= kFlushRel * (fabsf(l2 * a2) + fabsf(b0));
+    W0 = dry && z0 ? 0.f : W0;
+    W2 = dry && z2 ? 0.f : W2;
+  }
   const float n0 = ub - c, n2 = ub + c;
   const float he = hp + eps;
   const bool to_left = ((he * l2 - b0) * l2 < 0.f) && R.h < eps;
@@ -139,10 +157,12 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     fn += r0 * n0 + r2 * n2;
     ft += (r0 + r2) * vb;
   }
-  lm = bm;
+  // the reference's final flush of |d| < threshold, on the mass components
+  // (momentum below it changes no test: a cell that thin has no velocity)
+  lm = fabsf(bm) < eps ? 0.f : bm;
   ln = bn;
   lt = bt;
-  rm = fm;
+  rm = fabsf(fm) < eps ? 0.f : fm;
   rn = fn;
   rt = ft;
 }
