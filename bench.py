@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--spinup-s", type=float, default=3600.0,
+                    help="watershed workloads: simulated seconds every member is advanced before the autotune "
+                         "and warmup (they start dry; 3600 s is peak rain), in both forms, so the timed window "
+                         "and the CPU baseline see flowing water (0: time from the dry start)")
     ap.add_argument("--workload", default="ensemble", choices=["ensemble", "watershed", "thacker", "circular"])
     ap.add_argument("--form", default=None, choices=["f32", "f64"],
                     help="single-grid workloads: fused form (default f32; the ensemble is always f32)")
@@ -78,12 +82,14 @@ def interior(case) -> int:
 
 def cpu_baseline(cases, target_s):
     """The FP64 oracle (the reference's unfused kernel sequence restated in C
-    with OpenMP) on the host cores: the first steps of the workload's grids,
-    taken in order, until about target_s seconds of work are done."""
+    with OpenMP) on the host cores: the steps after each case's start (the
+    spun-up state for the watershed workloads) of the workload's grids, taken
+    in order, until about target_s seconds of work are done."""
     from oracle.pyoracle import Oracle, num_threads
     done_cells = 0
     spent = 0.0
     used = []
+    t_start = []
     for case in cases:
         if spent >= target_s:
             break
@@ -101,10 +107,12 @@ def cpu_baseline(cases, target_s):
         spent += dt
         done_cells += cells * int(r["steps"])
         used.append(f"{case.rows - 2}x{case.cols - 2}:{int(r['steps'])}")
+        t_start.append(float(case.scalars.get("sim_start", 0.0)))
+    starts = f"{min(t_start):g}" if min(t_start) == max(t_start) else f"{min(t_start):.0f}..{max(t_start):.0f}"
     return {"value": round(done_cells / spent / 1e9, 4), "unit": "GCUPS", "cores": num_threads(),
             "kind": "port",
-            "sample": f"FP64 oracle (C/OpenMP restatement of the reference's unfused kernels), first steps "
-                      f"from t=0 of grids [interior:steps] {', '.join(used)}; {spent:.1f} s"}
+            "sample": f"FP64 oracle (C/OpenMP restatement of the reference's unfused kernels), the steps "
+                      f"after t = {starts} s of grids [interior:steps] {', '.join(used)}; {spent:.1f} s"}
 
 
 def traffic_for(kernel: str, workload: str):
@@ -140,9 +148,16 @@ def run_ensemble(args):
     for m in members:
         batch.add(m)
     stream = torch.cuda.ExternalStream(batch.stream)
+    ids = sorted(batch.members)
+    t0 = time.perf_counter()
+    if args.spinup_s > 0:
+        batch.run(t_stop=args.spinup_s)
+    base = {i: batch.poll(i)["steps"] for i in ids}
+    log(f"spun up to t >= {args.spinup_s:g} s in {min(base.values())}..{max(base.values())} steps "
+        f"({time.perf_counter() - t0:.1f} s)")
     # the autotuner picks the launch shape on these members with real steps
     launch = batch.autotune(steps=8, cache=False)
-    tuned = batch.poll(sorted(batch.members)[0])["steps"]
+    tuned = batch.poll(ids[0])["steps"] - base[ids[0]]
     log(f"autotuned launch (waves per SIMD, graph steps) = {launch} in {tuned} steps")
     batch.run(max_steps=args.warmup)
     batch.arm(max_steps=args.steps)
@@ -155,10 +170,10 @@ def run_ensemble(args):
     e1.synchronize()
     wall = time.perf_counter() - w0
     batch.sync()
-    stats = [batch.status(i) for i in sorted(batch.members)]
-    steps_taken = {s["steps"] for s in stats}
-    if steps_taken != {tuned + args.warmup + args.steps} or any(s["finished"] for s in stats):
-        log(f"a member finished inside the timed window (steps {sorted(steps_taken)}); lower --steps")
+    stats = [batch.status(i) for i in ids]
+    advanced = {s["steps"] - base[i] for i, s in zip(ids, stats)}
+    if advanced != {tuned + args.warmup + args.steps} or any(s["finished"] for s in stats):
+        log(f"a member finished inside the timed window (steps {sorted(advanced)}); lower --steps")
         sys.exit(1)
     ms = e0.elapsed_time(e1)
     per_launch_s = ms / 1e3 / args.steps
@@ -174,7 +189,8 @@ def run_ensemble(args):
                    "interior_cells": cells, "sides": [m.rows - 2 for m in members], "horizon_h": 2.0,
                    "form": "fp32_batched (FP32 state, FP64 time and volume)", "boundaries": "open",
                    "waves_per_launch": batch.work_waves, "autotuned_launch": list(launch),
-                   "tuning_steps": tuned,
+                   "tuning_steps": tuned, "spinup_s": args.spinup_s,
+                   "spinup_steps_range": [min(base.values()), max(base.values())],
                    "simtime_s_range_after": [round(min(simt), 1), round(max(simt), 1)]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
@@ -182,17 +198,41 @@ def run_ensemble(args):
     }
     batch.close()
     fp64_steps = args.steps if args.fp64_steps is None else args.fp64_steps
+    spun = []
     if not args.no_fp64 and fp64_steps > 0:
-        res["fp64_plain"] = fp64_plain(members, res["config"]["tuning_steps"] + args.warmup, fp64_steps)
+        res["fp64_plain"] = fp64_plain(members, args.spinup_s, tuned + args.warmup, fp64_steps,
+                                       keep=spun if args.cpu_seconds > 0 else None)
         res["speedup_fp32_batched_vs_fp64_plain"] = round(gcups / res["fp64_plain"]["value"], 3)
-    res["cases"] = members
+    res["cases"] = spun or [spun_up(m, args.spinup_s) for m in members[:8]] if args.cpu_seconds > 0 else members
     return res
 
 
-def fp64_plain(members, warmup, steps):
+def restart_at(case, state, simtime):
+    """`case` with its state replaced by an FP64 state at `simtime` (the
+    reference's hotstart: sim_start and the initial rasters)."""
+    c = case.copy()
+    for k in ("h", "qx", "qy"):
+        c.arrays[k] = state[k][1:-1, 1:-1].copy() if state[k].shape != c.arrays[k].shape else state[k].copy()
+    c.scalars["sim_start"] = float(simtime)
+    return c
+
+
+def spun_up(case, t_stop):
+    """`case` advanced to t_stop by the FP64 fused form."""
+    import triton_amd as ta
+    if t_stop <= 0:
+        return case
+    with ta.Simulation(case, ta.FORM_F64_FUSED) as sim:
+        st = sim.run(t_stop=t_stop)
+        return restart_at(case, sim.state(), st["simtime"])
+
+
+def fp64_plain(members, spinup_s, warmup, steps, keep=None):
     """The same members stepped one at a time by the plain FP64 fused form
-    (one launch per step per member) over the same step window as the FP32
-    batch (after the same warmup), timed on each member's stream."""
+    (one launch per step per member) over the same window as the FP32 batch
+    (advanced to spinup_s, then the same number of untimed steps), timed on
+    each member's stream.  keep: a list that receives the first members'
+    spun-up states as cases (the CPU baseline's sample)."""
     import torch
     import triton_amd as ta
     total_ms = 0.0
@@ -200,6 +240,11 @@ def fp64_plain(members, warmup, steps):
     for m in members:
         sim = ta.Simulation(m, ta.FORM_F64_FUSED)
         stream = torch.cuda.ExternalStream(sim.stream)
+        if spinup_s > 0:
+            st = sim.run(t_stop=spinup_s)
+            if keep is not None and len(keep) < 8:
+                keep.append(restart_at(m, sim.state(), st["simtime"]))
+        base = int(sim.sync()["steps"])
         sim.run(max_steps=warmup)
         sim.arm(max_steps=steps)
         torch.cuda.synchronize()
@@ -209,7 +254,7 @@ def fp64_plain(members, warmup, steps):
         e1.record(stream)
         e1.synchronize()
         st = sim.sync()
-        if st["steps"] != warmup + steps:
+        if st["steps"] != base + warmup + steps or st["finished"]:
             log("an FP64 member finished inside its timed window")
             sys.exit(1)
         total_ms += e0.elapsed_time(e1)
@@ -217,7 +262,8 @@ def fp64_plain(members, warmup, steps):
         sim.close()
     g = total_cells / (total_ms / 1e3) / 1e9
     log(f"fp64 plain: {g:.2f} GCUPS over {len(members)} members x {steps} steps")
-    return {"value": round(g, 3), "unit": "GCUPS", "warmup": warmup, "steps_per_member": steps,
+    return {"value": round(g, 3), "unit": "GCUPS", "spinup_s": spinup_s, "warmup": warmup,
+            "steps_per_member": steps,
             "form": "fp64_fused, members one at a time", "kernel": "k_step_f64",
             "bytes_per_cell": b_alg(False, True, True)}
 
@@ -282,6 +328,10 @@ def run_single(args):
     cells = interior(case)
     sim = ta.Simulation(case, ta.FORM_F32_FUSED if f32 else ta.FORM_F64_FUSED)
     stream = torch.cuda.ExternalStream(sim.stream)
+    if args.workload == "watershed" and args.spinup_s > 0:
+        st = sim.run(t_stop=args.spinup_s)
+        desc["spinup_s"] = args.spinup_s
+        case = restart_at(case, sim.state(), st["simtime"])  # the CPU baseline's sample
     sim.run(max_steps=args.warmup)
     st0 = sim.sync()
     sim.arm(max_steps=args.steps)

@@ -175,7 +175,7 @@ typedef struct tr_tuning {
                               value; the reference's FP32 build uses 1e-6,
                               src/constants.h:43, which loses mass on
                               small-scale flows) */
-  int32_t waves_per_simd;  /* FP32 form: waves per SIMD of the launch (0: 4) */
+  int32_t waves_per_simd;  /* FP32 form: waves per SIMD of the launch (0: 5) */
 } tr_tuning;
 
 /* Allocate device state for `view` in the given form and upload it. */
@@ -217,7 +217,7 @@ int tr_dem_flow_route(const double *dem, int32_t nr, int32_t nc, double eps, dou
  * when the member would exceed the HBM budget given at creation. */
 typedef struct tr_batch tr_batch;
 /* Every launch cuts the active members into row bands, one band-strip per
- * wave, over at most waves_per_simd (0: 4) waves per SIMD of the device. */
+ * wave, over at most waves_per_simd (0: 5) waves per SIMD of the device. */
 int tr_batch_create(int64_t hbm_budget_bytes, int32_t waves_per_simd, tr_batch **out);
 /* Device bytes a member built from `v` would hold (admission control). */
 int64_t tr_batch_member_bytes(const tr_case_view *v);
@@ -239,7 +239,7 @@ int tr_batch_arm(tr_batch *b, double t_stop, int64_t max_steps);
 /* Queue nsteps batched steps without waiting (as captured hipGraphs of
  * graph_steps steps when tr_batch_set_launch enabled them). */
 int tr_batch_enqueue(tr_batch *b, int64_t nsteps);
-/* Launch shape: waves per SIMD of the row partition (0: 4) and steps per
+/* Launch shape: waves per SIMD of the row partition (0: 5) and steps per
  * captured hipGraph (0: eager launches).  Results do not depend on either. */
 int tr_batch_set_launch(tr_batch *b, int32_t waves_per_simd, int32_t graph_steps);
 /* Wait; members past their limits leave the launch.  n_active may be NULL. */

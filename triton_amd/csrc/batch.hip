@@ -80,12 +80,14 @@ constexpr int kVolBlocks = 256;
 // 0.1 m deep paraboloid fixture).
 constexpr float kDefaultDryEpsF32 = 1e-12f;
 
-// Waves per SIMD of a batched launch (the step kernel's registers allow 4,
-// and 4 measured fastest with the band partition) and the band heights a
+// Waves per SIMD of a batched launch (the step kernel's registers allow 5
+// resident, and 5 measured fastest on one large grid: more waves run in later
+// rounds that no longer share their neighbours' halo rows in L2; an uneven
+// ensemble gains from 10-15, which the autotuner finds) and the band heights a
 // small batch is given (it spreads over about one wave per SIMD: each piece
 // recomputes a 4-row halo, but a 400 x 400 grid steps in 28 us with 8-row
 // waves against 81 us with 48-row ones).
-constexpr int kWavesPerSimd = 4;
+constexpr int kWavesPerSimd = 5;
 constexpr int kFloorRowsPerWave = 8;
 constexpr int kMinRowsPerWave = 48;
 

@@ -60,12 +60,13 @@ struct Side {
 //  * the wave-1 part reduces to ub*(dqt - vb*dh) on the tangential momentum;
 //  * the dry-side tests hp - b0/l < -eps are evaluated as sign tests of
 //    ((hp+eps)*l - b0)*l, without the division;
-//  * both wet-wet clip bounds share one reciprocal;
+//  * the wet-wet clip bounds need a reciprocal only under an entropy fix;
 //  * the Harten-Hyman entropy fix runs only in waves where some lane needs it;
 //  * the front side is the total fluctuation minus the back side;
-//  * the final flush of |d| < threshold is kept for the mass components, and
-//    on faces with a dry side a relative flush of the two wave strengths
-//    removes FP32 round-off from balanced (zero) fluctuations;
+//  * the dry-side tests and the flushes run only on faces with a dry side:
+//    the final flush of |d| < threshold on the mass components, and a
+//    relative flush of the two wave strengths that removes FP32 round-off
+//    from balanced (zero) fluctuations;
 //  * a face with both sides dry (h = 0) needs no mask: with the Roe-average
 //    reciprocal kept finite every wave strength, source and fluctuation of
 //    it is an exact zero.
@@ -104,31 +105,39 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   const float hp = L.h + a0;
   const bool sub = l0 * l2 < 0.f && hp > 0.f && L.h > 0.f && R.h > 0.f;
   if (__any(sub)) {
-    const float d0 = l0 + e2, d2 = l2 + e0;
-    const float rr = rcpf(d0 * d2);
-    const float lo = l0 * d2 * rr * (a0 * l0 - L.h * hdxdt + e2 * R.h);
-    const float hi = l2 * d0 * rr * (-a2 * l2 + R.h * hdxdt + e0 * L.h);
+    // wet-wet clip; without an entropy fix the reference's factors
+    // l/(l+e) are exactly 1
+    float lo = a0 * l0 - L.h * hdxdt;
+    float hi = R.h * hdxdt - a2 * l2;
+    if (__any(fix0 || fix2)) {
+      lo = fix2 ? (lo + e2 * R.h) * (l0 * rcpf(l0 + e2)) : lo;
+      hi = fix0 ? (hi + e0 * L.h) * (l2 * rcpf(l2 + e0)) : hi;
+    }
     const float bc = fminf(fmaxf(b0, lo), hi);
     b0 = sub ? bc : b0;
   }
   float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
-  // Still water against a higher dry bed is balanced: both strengths are
-  // exact zeros, and the reference's final flush (|d| < 1e-12,
-  // src/kernels_flux.h:255-264) removes their FP64 round-off.  FP32
-  // round-off is ~1e-7 of the terms and would wet the dry cell by ~1e-10 m
-  // per step, enough to defeat the wet/dry test; on faces with a dry side a
-  // strength within kFlushRel of the terms it is formed from is zeroed.
+  const float n0 = ub - c, n2 = ub + c;
+  // Faces with a dry side (only they can send all mass one way) take the
+  // dry-side tests and the flushes; waves without one skip them.
   const bool dry = L.h < eps || R.h < eps;
-  if (__any(dry)) {
+  const bool any_dry = __any(dry);
+  bool to_left = false, to_right = false;
+  if (any_dry) {
+    // Still water against a higher dry bed is balanced: both strengths are
+    // exact zeros, and the reference's final flush (|d| < 1e-12,
+    // src/kernels_flux.h:255-264) removes their FP64 round-off.  FP32
+    // round-off is ~1e-7 of the terms and would wet the dry cell by ~1e-10 m
+    // per step, enough to defeat the wet/dry test; a strength within
+    // kFlushRel of the terms it is formed from is zeroed.
     const bool z0 = fabsf(W0) <= kFlushRel * (fabsf(l0 * a0) + fabsf(b0));
     const bool z2 = fabsf(W2) <= kFlushRel * (fabsf(l2 * a2) + fabsf(b0));
     W0 = dry && z0 ? 0.f : W0;
     W2 = dry && z2 ? 0.f : W2;
+    const float he = hp + eps;
+    to_left = ((he * l2 - b0) * l2 < 0.f) && R.h < eps;
+    to_right = !to_left && ((he * l0 - b0) * l0 < 0.f) && L.h < eps;
   }
-  const float n0 = ub - c, n2 = ub + c;
-  const float he = hp + eps;
-  const bool to_left = ((he * l2 - b0) * l2 < 0.f) && R.h < eps;
-  const bool to_right = !to_left && ((he * l0 - b0) * l0 < 0.f) && L.h < eps;
   // regular split: back side gets the waves with non-positive speed
   const float B0 = l0 > 0.f ? 0.f : W0, B2 = l2 > 0.f ? 0.f : W2;
   float bm = B0 + B2;
@@ -137,13 +146,14 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   const float tm = W0 + W2;
   float tn = W0 * n0 + W2 * n2;
   float tt = tm * vb + t1;
-  if (to_left || to_right) {
+  if (any_dry) {
     // all mass to the wet side, no momentum from the regular waves
-    bm = to_left ? tm : 0.f;
-    bn = 0.f;
-    bt = 0.f;
-    tn = 0.f;
-    tt = 0.f;
+    const bool one_way = to_left || to_right;
+    bm = to_left ? tm : (to_right ? 0.f : bm);
+    bn = one_way ? 0.f : bn;
+    bt = one_way ? 0.f : bt;
+    tn = one_way ? 0.f : tn;
+    tt = one_way ? 0.f : tt;
   }
   float fm = tm - bm, fn = tn - bn, ft = tt - bt;
   if (__any(fix0 || fix2)) {
@@ -157,12 +167,16 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     fn += r0 * n0 + r2 * n2;
     ft += (r0 + r2) * vb;
   }
-  // the reference's final flush of |d| < threshold, on the mass components
-  // (momentum below it changes no test: a cell that thin has no velocity)
-  lm = fabsf(bm) < eps ? 0.f : bm;
+  if (any_dry) {
+    // the reference's final flush of |d| < threshold, on the mass components
+    // (momentum below it changes no test: a cell that thin has no velocity)
+    bm = dry && fabsf(bm) < eps ? 0.f : bm;
+    fm = dry && fabsf(fm) < eps ? 0.f : fm;
+  }
+  lm = bm;
   ln = bn;
   lt = bt;
-  rm = fabsf(fm) < eps ? 0.f : fm;
+  rm = fm;
   rn = fn;
   rt = ft;
 }
@@ -189,6 +203,17 @@ struct Loc {
   int rows, cols, nz;
   float add0;
 };
+
+// A pointer every lane holds, made explicitly wave-uniform: the compiler
+// then keeps it in scalar registers for the whole piece instead of
+// re-reading it from a vector register at every row.
+template <class T>
+__device__ __forceinline__ T* uni(T* p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(v));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(v >> 32));
+  return reinterpret_cast<T*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
 
 struct Raw {
   float h, qx, qy, dzx, dzy, n2, add;
@@ -242,17 +267,17 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   const TR_GLOBAL int* __restrict__ src_row = gptr(P.src_row);
   const TR_GLOBAL int* __restrict__ src_col = gptr(P.src_col);
   const TR_GLOBAL double* __restrict__ src_add = gptr(P.src_add + cur * nsrc);
-  TR_GLOBAL float* __restrict__ ho = gptr(P.h[cur ^ 1]);
-  TR_GLOBAL float* __restrict__ qxo = gptr(P.qx[cur ^ 1]);
-  TR_GLOBAL float* __restrict__ qyo = gptr(P.qy[cur ^ 1]);
+  TR_GLOBAL float* __restrict__ ho = uni(gptr(P.h[cur ^ 1]));
+  TR_GLOBAL float* __restrict__ qxo = uni(gptr(P.qx[cur ^ 1]));
+  TR_GLOBAL float* __restrict__ qyo = uni(gptr(P.qy[cur ^ 1]));
   TR_GLOBAL float* __restrict__ maxh = gptr(P.maxh);
   Loc L;
-  L.h = gptr(P.h[cur]);
-  L.qx = gptr(P.qx[cur]);
-  L.qy = gptr(P.qy[cur]);
-  L.dzx = gptr(P.dzx);
-  L.dzy = gptr(P.dzy);
-  L.n2 = gptr(P.nman2);
+  L.h = uni(gptr(P.h[cur]));
+  L.qx = uni(gptr(P.qx[cur]));
+  L.qy = uni(gptr(P.qy[cur]));
+  L.dzx = uni(gptr(P.dzx));
+  L.dzy = uni(gptr(P.dzy));
+  L.n2 = uni(gptr(P.nman2));
   L.rid = gptr(P.runoff_id);
   L.nz = P.num_runoffs;
   L.zadd = gptr(P.num_runoffs > 0 ? P.zone_add + cur * P.num_runoffs : nullptr);
@@ -294,38 +319,14 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
     const float u = mv ? s.qx * ih : 0.f;
     const float v = mv ? s.qy * ih : 0.f;
 
-    // x face (c, c+1) of row r: back side stays in this lane (rhs*0), the
-    // front side goes to lane c+1 (rhs*1).  Faces outside the reference's
-    // ranges only feed ghost or halo cells that are never updated, so they
-    // are computed unmasked; fluctuations are scaled by dt/dx at the update.
-    float c0h, c0x, c0y, c1h, c1x, c1y;
-    {
-      Side Rt;
-      Rt.h = from_right(s.h);
-      Rt.qn = from_right(s.qx);
-      Rt.qt = from_right(s.qy);
-      Rt.un = from_right(u);
-      Rt.ut = from_right(v);
-      Rt.sq = from_right(sq);
-      const Side Lt{s.h, s.qx, s.qy, u, v, sq};
-      float em, en, et;
-      face32(Lt, Rt, s.dzx, hdxdt, eps, c0h, c0x, c0y, em, en, et);
-      c1h = from_left(em);
-      c1x = from_left(en);
-      c1y = from_left(et);
-    }
-
-    // y face between row r (south, back) and row r-1 (north, front)
-    float y1h, y1x, y1y;
+    // y face between row r (south, back) and row r-1 (north, front); its
+    // back side joins row r's x back side below, its front side completes
+    // row r-1, which is updated before row r's x face is formed
+    float y1h, y1x, y1y, ylm, yln, ylt;
     {
       const Side S{s.h, s.qy, s.qx, v, u, sq};
       const Side N{ph, pqy, pqx, pv, pu, psq};
-      float lm, ln, lt;
-      face32(S, N, s.dzy, hdxdt, eps, lm, ln, lt, y1h, y1y, y1x);
-      // rhs*0 of row r = x back side + y back side (reference order)
-      c0h += lm;
-      c0y += ln;
-      c0x += lt;
+      face32(S, N, s.dzy, hdxdt, eps, ylm, yln, ylt, y1h, y1y, y1x);
     }
 
     // update row r-1
@@ -359,6 +360,32 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
       }
       nh = hnew;
     }
+    // x face (c, c+1) of row r: back side stays in this lane (rhs*0), the
+    // front side goes to lane c+1 (rhs*1).  Faces outside the reference's
+    // ranges only feed ghost or halo cells that are never updated, so they
+    // are computed unmasked; fluctuations are scaled by dt/dx at the update.
+    float c0h, c0x, c0y, c1h, c1x, c1y;
+    {
+      Side Rt;
+      Rt.h = from_right(s.h);
+      Rt.qn = from_right(s.qx);
+      Rt.qt = from_right(s.qy);
+      Rt.un = from_right(u);
+      Rt.ut = from_right(v);
+      Rt.sq = from_right(sq);
+      const Side Lt{s.h, s.qx, s.qy, u, v, sq};
+      float em, en, et;
+      face32(Lt, Rt, s.dzx, hdxdt, eps, c0h, c0x, c0y, em, en, et);
+      c1h = from_left(em);
+      c1x = from_left(en);
+      c1y = from_left(et);
+    }
+
+    // rhs*0 of row r = x back side + y back side (reference order)
+    c0h += ylm;
+    c0y += yln;
+    c0x += ylt;
+
     if (nsrc > 0 && ru_int) {
       while (ks < nsrc && src_row[ks] < ru) ks++;
       for (int k = ks; k < nsrc && src_row[k] == ru; k++)
@@ -423,7 +450,8 @@ __device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float best, int la
 
 // Batched step: wave w (placed XCD-contiguously) walks its pieces in order
 // and publishes one dt min per run of pieces of the same member.
-__global__ __launch_bounds__(64) void k_step_f32(BatchArgs a) {
+// 5 waves per SIMD: the step fits in 96 VGPRs without spills.
+__global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const int w = xcd_slot(int(blockIdx.x), a.nwaves);
   const int lane = threadIdx.x;
   const int p0 = a.wave_first[w], p1 = a.wave_first[w + 1];

@@ -14,11 +14,12 @@ import math
 import os
 from pathlib import Path
 
-# (waves per SIMD, steps per graph).  Up to 4 waves per SIMD are resident at
-# once (the step kernel's registers); more queue behind them, which evens out
-# uneven pieces (an ensemble of mixed sizes) but adds a tail when the pieces
-# are equal (one large grid).
-DEFAULT_CANDIDATES = ((4, 0), (6, 0), (8, 0))
+# (waves per SIMD, steps per graph).  Up to 5 waves per SIMD are resident at
+# once (the step kernel's registers); whole multiples of 5 queue behind them,
+# which evens out uneven pieces (an ensemble of mixed sizes) but costs L2
+# sharing between neighbouring bands when the pieces are equal (one large
+# grid).
+DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0))
 
 
 def cache_path() -> Path:
