@@ -80,6 +80,9 @@ typedef struct tr_case_view {
   int32_t open_boundaries;
   double sim_start, sim_duration, print_interval, print_observation, init_dt;
   int32_t time_series_flag, checkpoint_id, it_print;
+  int32_t it_count;              /* iterations before the start (hotstart .cfg) */
+  int32_t num_obs;               /* observation points (time_series_flag) */
+  int32_t *obs_pos;              /* [num_obs] flat padded cell index */
 } tr_case_view;
 
 /* Parse a .cfg and load every input it names (src/triton_init.h:33-81).
@@ -154,6 +157,10 @@ int tr_compute_extbc_values_f64(int32_t size, int32_t nrows, int32_t ncols, doub
 int tr_wet_dry_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, const double *h,
                    double *qx, double *qy, const double *dem, double *maxh, double hextra,
                    int32_t mpi_tasks, void *stream);
+/* src/kernels.h:521-535 Kernels::copy_observation_points:
+ * obs_arr[i] = arr[relative_index[i]] for i < size (device pointers). */
+int tr_copy_observation_points_f64(int32_t size, const double *arr, double *obs_arr,
+                                   const int32_t *relative_index, void *stream);
 
 /* ---- device-resident simulation ------------------------------------------ */
 typedef struct tr_sim tr_sim;
@@ -166,6 +173,13 @@ typedef struct tr_sim_status {
   int32_t finished;     /* simtime >= sim_duration */
   double volume;        /* FP64 water volume over interior cells (m^3) */
   double volume_in;     /* FP64 integrated rain + inflow volume (m^3) */
+  int32_t observed;     /* the last step completed an observation interval
+                           (src/triton_simulate.h:78-91) */
+  int32_t reserved;
+  double last_obs;      /* time of the last observation interval boundary */
+  double average_dt;    /* sum of dt since the last it_print boundary: what the
+                           reference writes as time_step= (src/output.h:755-811) */
+  int64_t it_count;     /* iterations, those before a hotstart included */
 } tr_sim_status;
 
 /* Tuning of one simulation / batch member (zero fields take the defaults). */
@@ -195,6 +209,14 @@ int tr_sim_enqueue(tr_sim *s, int64_t nsteps);
 int tr_sim_sync(tr_sim *s, tr_sim_status *st);
 /* Copy the current state to host (padded, FP64). Any pointer may be NULL. */
 int tr_sim_download(tr_sim *s, double *h, double *qx, double *qy, double *maxh);
+/* h, qx, qy of the current state at `n` flat padded cell indices (host
+ * arrays; any output may be NULL): the device gather of
+ * src/triton_simulate.h get_observation_data_to_host. */
+int tr_sim_sample(tr_sim *s, int32_t n, const int32_t *pos, double *h, double *qx, double *qy);
+/* on != 0: tr_sim_run stops after each step that completes an observation
+ * interval (status.observed), so the caller can sample it as the reference's
+ * loop does; the next run continues. */
+int tr_sim_set_obs_halt(tr_sim *s, int32_t on);
 /* The hipStream_t the simulation runs on. */
 void *tr_sim_stream(tr_sim *s);
 /* Bytes of device memory held. */
@@ -251,6 +273,11 @@ int tr_batch_status(tr_batch *b, int32_t id, tr_sim_status *st);
  * the control blocks (no device work; volume is NaN). */
 int tr_batch_poll(tr_batch *b, int32_t id, tr_sim_status *st);
 int tr_batch_download(tr_batch *b, int32_t id, double *h, double *qx, double *qy, double *maxh);
+/* tr_sim_sample and tr_sim_set_obs_halt for one member (the halt holds the
+ * member until the next tr_batch_arm). */
+int tr_batch_sample(tr_batch *b, int32_t id, int32_t n, const int32_t *pos, double *h, double *qx,
+                    double *qy);
+int tr_batch_set_obs_halt(tr_batch *b, int32_t id, int32_t on);
 int64_t tr_batch_device_bytes(const tr_batch *b);
 int64_t tr_batch_member_cells(tr_batch *b, int32_t id);
 int32_t tr_batch_work_waves(const tr_batch *b);

@@ -35,12 +35,16 @@ class OrcCase(C.Structure):
         ("sim_start", C.c_double), ("sim_duration", C.c_double),
         ("print_interval", C.c_double), ("print_observation", C.c_double),
         ("time_series_flag", C.c_int), ("checkpoint_id", C.c_int),
+        ("num_obs", C.c_int), ("obs_pos", _p_i), ("obs_rec", _p_d), ("obs_cap", C.c_int),
+        ("obs_rows", C.POINTER(C.c_int)),
+        ("it_print", C.c_int), ("it_count0", C.c_int),
     ]
 
 
 class OrcResult(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("simtime", C.c_double), ("last_dt", C.c_double),
-                ("print_id", C.c_int), ("init_dt", C.c_double)]
+                ("print_id", C.c_int), ("init_dt", C.c_double), ("last_obs", C.c_double),
+                ("average_dt", C.c_double), ("it_count", C.c_longlong)]
 
 
 _LIB = None
@@ -107,21 +111,43 @@ class Oracle:
         c.print_observation = float(self.s.get("print_observation", c.print_interval))
         c.time_series_flag = int(self.s.get("time_series_flag", 0))
         c.checkpoint_id = int(self.s.get("checkpoint_id", 0))
+        c.it_print = int(self.s.get("it_print", 10000))
+        c.it_count0 = int(self.s.get("it_count", 0))
+        self.obs = None
+        self._obs_rows = C.c_int(0)
+        if c.time_series_flag and "obs_pos" in a and a["obs_pos"].shape[0] > 0:
+            c.num_obs = a["obs_pos"].shape[0]
+            c.obs_pos = _ip(a["obs_pos"])
+            c.obs_rows = C.pointer(self._obs_rows)
         self.c = c
 
     @classmethod
     def from_case(cls, case) -> "Oracle":
         return cls(case.arrays, case.scalars)
 
-    def run(self, t_stop: float = 0.0, max_steps: int = -1) -> dict:
+    def run(self, t_stop: float = 0.0, max_steps: int = -1, obs_cap: int = 0) -> dict:
         """Run the reference loop from the case start while simtime < t_stop
-        (0: sim_duration) or for max_steps steps; state is advanced in place."""
+        (0: sim_duration) or for max_steps steps; state is advanced in place.
+        With observation points and obs_cap > 0, self.obs receives the rows
+        (simtime, h[obs], qx[obs], qy[obs]) the reference writes."""
         r = OrcResult()
+        rec = None
+        if self.c.num_obs > 0 and obs_cap > 0:
+            rec = np.zeros((obs_cap, 1 + 3 * self.c.num_obs), np.float64)
+            self.c.obs_rec, self.c.obs_cap = _dp(rec), obs_cap
+        else:
+            self.c.obs_rec, self.c.obs_cap = C.cast(None, _p_d), 0
         rc = lib().orc_run(C.byref(self.c), float(t_stop), int(max_steps), C.byref(r))
         if rc != 0:
             raise RuntimeError(f"orc_run failed with {rc}")
+        if rec is not None:
+            n = self._obs_rows.value
+            if n > obs_cap:
+                raise RuntimeError(f"{n} observation rows exceed obs_cap {obs_cap}")
+            self.obs = rec[:n].copy()
         return {"steps": r.steps, "simtime": r.simtime, "dt": r.last_dt, "print_id": r.print_id,
-                "init_dt": r.init_dt}
+                "init_dt": r.init_dt, "last_obs": r.last_obs, "average_dt": r.average_dt,
+                "it_count": r.it_count}
 
     @property
     def state(self) -> dict:

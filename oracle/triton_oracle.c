@@ -436,6 +436,23 @@ static double init_dt_of(const orc_case *c) {
 /* src/triton_simulate.h:54-134 with compute_local_dt (:191-198),
  * compute_global_dt (:202-226) and compute_new_state (:230-346); the runoff
  * row starts where src/triton_init.h:1299-1311 puts it. */
+/* One observation row: simtime and the state at the observation cells. */
+static void obs_row(const orc_case *c, double simtime) {
+  if (!c->time_series_flag || !c->obs_rec || !c->obs_rows) return;
+  int k = *c->obs_rows;
+  if (k < c->obs_cap) {
+    double *row = c->obs_rec + (size_t)k * (size_t)(1 + 3 * c->num_obs);
+    row[0] = simtime;
+    for (int i = 0; i < c->num_obs; i++) {
+      int id = c->obs_pos[i];
+      row[1 + i] = c->h[id];
+      row[1 + c->num_obs + i] = c->qx[id];
+      row[1 + 2 * c->num_obs + i] = c->qy[id];
+    }
+  }
+  *c->obs_rows = k + 1;
+}
+
 int orc_run(const orc_case *c, double t_stop, long long max_steps, orc_result *res) {
   const int rows = c->rows, cols = c->cols, n = rows * cols;
   if (c->num_runoffs > 0 && c->roff_nrows < 2) return 2;
@@ -460,6 +477,9 @@ int orc_run(const orc_case *c, double t_stop, long long max_steps, orc_result *r
   int idx_low = 0;
   long long steps = 0;
   double dt = 0.0;
+  double last_obs = simtime, average_dt = 0.0;  /* src/triton_simulate.h:42,47 */
+  if (c->obs_rows) *c->obs_rows = 0;
+  obs_row(c, simtime);                          /* :31-37 */
 
   const double stop = (t_stop > 0.0 && t_stop < c->sim_duration) ? t_stop : c->sim_duration;
   while (simtime < stop) {
@@ -510,8 +530,16 @@ int orc_run(const orc_case *c, double t_stop, long long max_steps, orc_result *r
     orc_wet_dry(rows, cols, c->h, c->qx, c->qy, c->dem, c->maxh, c->hextra);
 
     simtime += dt;
+    average_dt += dt;                                           /* :66 */
+    if (c->it_print > 0 && ((long long)c->it_count0 + steps) % c->it_print == 0)
+      average_dt = 0.0;                                         /* :69-75 */
+    if (c->time_series_flag && simtime - last_obs >= c->print_observation) {
+      last_obs += c->print_observation;                         /* :78-91 */
+      obs_row(c, simtime);
+    }
     if (simtime >= c->print_interval * (print_id + 1)) print_id++;
   }
+  if (!(simtime < c->sim_duration)) obs_row(c, simtime);       /* :113-121 */
   free(buf);
   if (res) {
     res->steps = steps;
@@ -519,6 +547,9 @@ int orc_run(const orc_case *c, double t_stop, long long max_steps, orc_result *r
     res->last_dt = dt;
     res->print_id = print_id;
     res->init_dt = init_dt;
+    res->last_obs = last_obs;
+    res->average_dt = average_dt;
+    res->it_count = (long long)c->it_count0 + steps;
   }
   return 0;
 }

@@ -28,6 +28,9 @@
  *       pinned by an exact uniform-flow steady state, the symmetries above
  *       with open boundaries, and a y-uniform dam break that must stay
  *       y-uniform while water leaves through the east edge.
+ *   orc_run observation rows and the time_step= dt sum: no reference
+ *       fixture; tests/test_oracle_invariants.py derives the recorded steps
+ *       and the sum again in Python from a run that records every step.
  *   orc_compute_extbc_values types 0/1/2: parity unpinned (no fixture, no
  *       closed form beyond the defining formula).
  */
@@ -66,6 +69,20 @@ typedef struct orc_case {
   /* time control (src/triton_simulate.h:12-226) */
   double sim_start, sim_duration, print_interval, print_observation;
   int time_series_flag, checkpoint_id;
+  /* observation points (src/triton_simulate.h:31-37, 78-91, 113-121): with
+   * time_series_flag and obs_rec set, a run from the case start appends rows
+   * (simtime, h[obs], qx[obs], qy[obs]) -- the start, every step that
+   * completes a print_observation interval, and the end when the run reaches
+   * sim_duration.  obs_rec holds obs_cap rows of 1 + 3*num_obs doubles;
+   * *obs_rows counts every row, also those that did not fit. */
+  int num_obs;
+  const int *obs_pos;
+  double *obs_rec;
+  int obs_cap;
+  int *obs_rows;
+  /* log cadence and iterations before the start: the dt sum the reference
+   * writes as time_step= (src/triton_simulate.h:41-75) */
+  int it_print, it_count0;
 } orc_case;
 
 typedef struct orc_result {
@@ -74,6 +91,9 @@ typedef struct orc_result {
   double last_dt;             /* dt of the last step */
   int print_id;               /* prints reached */
   double init_dt;             /* all-dry fallback dt used by the run */
+  double last_obs;            /* last observation interval boundary */
+  double average_dt;          /* dt sum since the last it_print boundary */
+  long long it_count;         /* it_count0 + steps */
 } orc_result;
 
 /* Kernel restatements (argument meaning as in the reference). */

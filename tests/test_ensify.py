@@ -51,7 +51,8 @@ def test_ensemble_paraboloid_job(gpu, ref_tree, tmp_path, form):
     root = build_root(ref_tree, tmp_path)
     ms = ensify.run(root / "ensembles_ensify.yaml", base_dir=root, form=form)
     assert all(m.status["finished"] and m.status["print_id"] == 12 for m in ms)
-    assert all(len(m.outputs) == 12 * 3 * 2 for m in ms)  # 12 prints x H,QX,QY x asc+bin
+    # 12 prints x (H,QX,QY x asc+bin + the hotstart .cfg)
+    assert all(len(m.outputs) == 12 * (3 * 2 + 1) for m in ms)
     assert (root / "ens1.out").read_text().startswith("args: ./paraboloid_ens1.cfg")
     for var in ("H", "QX", "QY"):
         a = (root / "output_ens1" / "asc" / f"{var}_12_00.out").read_bytes()

@@ -17,6 +17,11 @@ index, a wrong neighbour) would break.
   Thacker            the planar oscillation in a paraboloid (SWASHES 4.2.2)
                      after a quarter period: depth error and the motion of
                      the water's centroid against the analytic solution.
+  observations       the rows the oracle records for an observation interval
+                     are the ones a Python restatement of the reference's
+                     test picks out of a run that records every step; the
+                     time_step= dt sum matches the per-step dts.
+  hotstart .cfg      the reference's key replacement, including its quirks.
 """
 import math
 
@@ -128,3 +133,80 @@ def test_thacker_quarter_period_against_analytic():
     assert np.hypot(*(cn - ce)) < 0.02
     assert np.hypot(*(c0 - ce)) > 0.6
     assert math.isclose(ce[1], c0[0], rel_tol=1e-6)
+
+
+def obs_case(po, it_print=7):
+    nr, nc = 24, 30
+    y, x = np.mgrid[0:nr, 0:nc]
+    dem = 0.05 * np.sin(x / 4.0) + 0.03 * y / nr
+    h = np.where((x - 10) ** 2 + (y - 12) ** 2 < 25, 1.0, 0.3)
+    c = Case.from_arrays(dem=dem, h=h, dx=1.0, courant=0.5, hextra=1e-3, sim_duration=4.0,
+                         print_interval=1.5, time_series_flag=1, print_observation=po,
+                         observations=dict(rows=[3, 12, 20], cols=[4, 10, 29]))
+    c.scalars["it_print"] = it_print
+    return c
+
+
+def test_observation_rows_follow_the_reference_test():
+    every = Oracle.from_case(obs_case(0.0))
+    r_all = every.run(obs_cap=4096)
+    rows = every.obs
+    n_steps = int(r_all["steps"])
+    # print_observation = 0: the start, every step, and the end
+    assert rows.shape[0] == n_steps + 2
+    times = rows[1:-1, 0]
+    for po in (0.37, 0.5, 1.1):
+        o = Oracle.from_case(obs_case(po))
+        r = o.run(obs_cap=4096)
+        assert r["steps"] == n_steps
+        # src/triton_simulate.h:78-81, restated: a step is recorded when its
+        # simtime has moved po past the last boundary, which advances by po
+        last, pick = 0.0, [0]
+        for k, t in enumerate(times):
+            if t - last >= po:
+                last += po
+                pick.append(k + 1)
+        pick.append(rows.shape[0] - 1)
+        np.testing.assert_array_equal(o.obs, rows[pick])
+        assert r["last_obs"] == last
+
+
+def test_time_step_sum_restarts_at_it_print():
+    case = obs_case(0.5, it_print=5)
+    dts = []
+    for k in range(1, 13):
+        dts.append(Oracle.from_case(case).run(max_steps=k)["dt"])
+    acc = 0.0
+    for k, dt in enumerate(dts, start=1):
+        acc += dt
+        if k % 5 == 0:
+            acc = 0.0
+        r = Oracle.from_case(case).run(max_steps=k)
+        assert r["average_dt"] == acc and r["it_count"] == k
+
+
+def test_cfg_for_print_replacement():
+    from triton_amd.output import cfg_for_print
+    text = ("# sim_start_time= is replaced in this comment first\n"
+            "sim_start_time=0\n"
+            "checkpoint_id=0\n"
+            "time_step=0.01\n"
+            "it_count=0\n"
+            "print_interval=1.5\n")
+    out = cfg_for_print(text, 2.5, 3, 0.125, 40)
+    # the reference replaces the first occurrence of the key over the length
+    # of the line that matched (src/output.h:760-767).  The comment line
+    # (51 characters) holds the first occurrence at offset 2: its replacement
+    # runs over the rest of the comment, the newline and the 's' of the next
+    # line; the next line's own turn (16 characters) then overwrites the
+    # first 16 characters of the inserted text again.
+    assert out.splitlines() == [
+        "# sim_start_time=2.50000000000000000.50000000000000000im_start_time=0",
+        "checkpoint_id=3",
+        "time_step=0.12500000000000000",
+        "it_count=40",
+        "print_interval=1.5",
+    ]
+    plain = "sim_start_time=0\ncheckpoint_id=0\ntime_step=0.01\nit_count=0\n"
+    assert cfg_for_print(plain, 1.0 / 3.0, 1, 0.0, 7) == (
+        "sim_start_time=0.33333333333333331\ncheckpoint_id=1\ntime_step=0.00000000000000000\nit_count=7\n")

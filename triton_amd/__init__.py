@@ -64,6 +64,7 @@ class CaseView(C.Structure):
         ("print_interval", C.c_double), ("print_observation", C.c_double),
         ("init_dt", C.c_double),
         ("time_series_flag", C.c_int32), ("checkpoint_id", C.c_int32), ("it_print", C.c_int32),
+        ("it_count", C.c_int32), ("num_obs", C.c_int32), ("obs_pos", _p_i),
     ]
 
 
@@ -78,6 +79,8 @@ class SimStatus(C.Structure):
         ("simtime", C.c_double), ("dt", C.c_double), ("steps", C.c_int64),
         ("print_id", C.c_int32), ("finished", C.c_int32),
         ("volume", C.c_double), ("volume_in", C.c_double),
+        ("observed", C.c_int32), ("reserved", C.c_int32),
+        ("last_obs", C.c_double), ("average_dt", C.c_double), ("it_count", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -117,6 +120,11 @@ def lib() -> C.CDLL:
         "tr_sim_enqueue": (C.c_int, [vp, i64]),
         "tr_sim_sync": (C.c_int, [vp, C.POINTER(SimStatus)]),
         "tr_sim_download": (C.c_int, [vp, _p_d, _p_d, _p_d, _p_d]),
+        "tr_sim_sample": (C.c_int, [vp, i32, _p_i, _p_d, _p_d, _p_d]),
+        "tr_sim_set_obs_halt": (C.c_int, [vp, i32]),
+        "tr_batch_sample": (C.c_int, [vp, i32, i32, _p_i, _p_d, _p_d, _p_d]),
+        "tr_batch_set_obs_halt": (C.c_int, [vp, i32, i32]),
+        "tr_copy_observation_points_f64": (C.c_int, [i32, vp, vp, vp, vp]),
         "tr_sim_stream": (vp, [vp]),
         "tr_sim_device_bytes": (i64, [vp]),
         "tr_sim_destroy": (None, [vp]),
@@ -179,13 +187,15 @@ class Case:
 
     ARRAYS_F = ("h", "qx", "qy", "dem", "nman2", "maxh")
     # .cfg keys kept verbatim for the output writers
-    CONFIG_KEYS = ("output_folder", "output_format", "outfile_pattern", "print_option", "output_option")
+    CONFIG_KEYS = ("output_folder", "output_format", "outfile_pattern", "print_option", "output_option",
+                   "max_value_print_option")
 
     def __init__(self):
         self._handle = None
         self.arrays: dict[str, np.ndarray] = {}
         self.scalars: dict = {}
         self.config: dict[str, str] = {}
+        self.cfg_text: str | None = None  # the .cfg as read (written back at prints)
 
     # -- construction -----------------------------------------------------
     @classmethod
@@ -218,15 +228,19 @@ class Case:
                 self.arrays[nm] = np.ctypeslib.as_array(getattr(v, nm), shape=(k,)).copy()
             self.arrays["bc_var1"] = np.ctypeslib.as_array(v.bc_var1, shape=(v.bc_nvars,)).copy()
             self.arrays["bc_var2"] = np.ctypeslib.as_array(v.bc_var2, shape=(v.bc_nvars,)).copy()
+        if v.num_obs > 0:
+            self.arrays["obs_pos"] = np.ctypeslib.as_array(v.obs_pos, shape=(v.num_obs,)).copy()
         for k in ("dx", "xll", "yll", "courant", "hextra", "roff_row0", "open_boundaries",
                   "sim_start", "sim_duration", "print_interval", "print_observation",
-                  "init_dt", "time_series_flag", "checkpoint_id", "it_print"):
+                  "init_dt", "time_series_flag", "checkpoint_id", "it_print", "it_count"):
             self.scalars[k] = getattr(v, k)
         assert n == self.arrays["h"].size
         buf = C.create_string_buffer(4096)
         for k in cls.CONFIG_KEYS:
             if L.tr_case_config_value(h, k.encode(), buf, len(buf)) >= 0:
                 self.config[k] = buf.value.decode()
+        with open(cfg_path, encoding="utf-8", errors="replace") as f:
+            self.cfg_text = f.read()
         L.tr_case_free(h)
         self._handle = None
         return self
@@ -235,13 +249,15 @@ class Case:
     def from_arrays(cls, *, dem, h, qx=None, qy=None, nman=None, dx=1.0, courant=0.5,
                     hextra=1e-3, sim_duration=1.0, print_interval=None, open_boundaries=0,
                     walls=True, runoff=None, sources=None, track_maxh=False,
-                    time_series_flag=0, print_observation=None) -> "Case":
+                    time_series_flag=0, print_observation=None, observations=None) -> "Case":
         """Build a case from unpadded (nrows x ncols) arrays the way the
         reference's init pads them (src/triton_init.h:195-355): ghost ring,
         walls (bed 1e6) or ghost copies, squared Manning n.
 
         runoff: dict(time_s=[...], rate_ms=[[zone rows]], zone=int array) or None.
         sources: dict(rows=[...], cols=[...] (unpadded), time_s=[...], q=[[t x src]]) or None.
+        observations: dict(rows=[...], cols=[...]) (unpadded) sampled every
+        print_observation seconds when time_series_flag is set.
         """
         dem = np.asarray(dem, np.float64)
         nr, nc = dem.shape
@@ -300,12 +316,15 @@ class Case:
             self.arrays["hyg_val"] = np.ascontiguousarray(hv)
             self.arrays["src_pos"] = pos.astype(np.int32)
             init_dt = min(init_dt, ht[1] - ht[0])
+        if observations is not None and time_series_flag:
+            self.arrays["obs_pos"] = ((np.asarray(observations["rows"]) + 1) * cols
+                                      + np.asarray(observations["cols"]) + 1).astype(np.int32)
         self.scalars.update(dx=float(dx), xll=0.0, yll=0.0, courant=float(courant),
                             hextra=float(hextra), open_boundaries=int(open_boundaries),
                             sim_start=0.0, sim_duration=float(sim_duration), print_interval=pi,
                             print_observation=po, init_dt=init_dt * 0.01,
                             time_series_flag=int(time_series_flag), checkpoint_id=0,
-                            it_print=10000)
+                            it_print=10000, it_count=0)
         self.scalars.setdefault("roff_row0", 0)
         return self
 
@@ -329,8 +348,12 @@ class Case:
         for k in ("dx", "xll", "yll", "courant", "hextra", "sim_start", "sim_duration",
                   "print_interval", "print_observation", "init_dt"):
             setattr(v, k, float(s.get(k, 0.0)))
-        for k in ("open_boundaries", "time_series_flag", "checkpoint_id", "it_print", "roff_row0"):
+        for k in ("open_boundaries", "time_series_flag", "checkpoint_id", "it_print", "roff_row0",
+                  "it_count"):
             setattr(v, k, int(s.get(k, 0)))
+        if "obs_pos" in a:
+            v.num_obs = a["obs_pos"].shape[0]
+            v.obs_pos = _ptr(a["obs_pos"], C.c_int32)
         for k in self.ARRAYS_F:
             setattr(v, k, _ptr(a.get(k), C.c_double))
         if "roff_time" in a:
@@ -358,7 +381,14 @@ class Case:
         c = Case()
         c.arrays = {k: v.copy() for k, v in self.arrays.items()}
         c.scalars = dict(self.scalars)
+        c.config = dict(self.config)
+        c.cfg_text = self.cfg_text
         return c
+
+    @property
+    def obs_pos(self) -> np.ndarray:
+        """Flat padded indices of the observation cells (empty without)."""
+        return self.arrays.get("obs_pos", np.zeros(0, np.int32))
 
 
 class Simulation:
@@ -418,6 +448,19 @@ class Simulation:
         out = {k: np.empty(shape, np.float64) for k in ("h", "qx", "qy", "maxh")}
         check(lib().tr_sim_download(self._h, *(_ptr(out[k], C.c_double) for k in ("h", "qx", "qy", "maxh"))))
         return out
+
+    def sample(self, pos=None) -> dict[str, np.ndarray]:
+        """h, qx, qy of the current state at flat padded cell indices
+        (default: the case's observation cells)."""
+        pos = np.ascontiguousarray(self.case.obs_pos if pos is None else pos, dtype=np.int32)
+        out = {k: np.empty(pos.shape[0], np.float64) for k in ("h", "qx", "qy")}
+        check(lib().tr_sim_sample(self._h, int(pos.shape[0]), _ptr(pos, C.c_int32),
+                                  *(_ptr(out[k], C.c_double) for k in ("h", "qx", "qy"))))
+        return out
+
+    def set_obs_halt(self, on: bool = True) -> None:
+        """Stop run() after each step that completes an observation interval."""
+        check(lib().tr_sim_set_obs_halt(self._h, int(bool(on))))
 
     @property
     def device_bytes(self) -> int:
@@ -504,6 +547,21 @@ class Batch:
         from .tune import autotune_batch
         return autotune_batch(self, candidates=candidates, steps=steps, cache=cache)
 
+    def set_obs_halt(self, mid: int, on: bool = True) -> None:
+        """Stop the member after each step that completes an observation
+        interval, until the next arm()."""
+        check(lib().tr_batch_set_obs_halt(self._h, int(mid), int(bool(on))))
+
+    def sample(self, mid: int, pos=None) -> dict[str, np.ndarray]:
+        """h, qx, qy of a member's current state at flat padded cell indices
+        (default: its observation cells)."""
+        case = self.members[mid]
+        pos = np.ascontiguousarray(case.obs_pos if pos is None else pos, dtype=np.int32)
+        out = {k: np.empty(pos.shape[0], np.float64) for k in ("h", "qx", "qy")}
+        check(lib().tr_batch_sample(self._h, int(mid), int(pos.shape[0]), _ptr(pos, C.c_int32),
+                                    *(_ptr(out[k], C.c_double) for k in ("h", "qx", "qy"))))
+        return out
+
     def set_print_halt(self, mid: int, on: bool = True) -> None:
         """Stop the member after each step that completes a print interval,
         until the next arm(), so its state can be written at print times."""
@@ -518,9 +576,10 @@ class Batch:
     def state(self, mid: int) -> dict[str, np.ndarray]:
         case = self.members[mid]
         shape = (case.rows, case.cols)
-        out = {k: np.empty(shape, np.float64) for k in ("h", "qx", "qy")}
+        keys = ("h", "qx", "qy", "maxh") if "maxh" in case.arrays else ("h", "qx", "qy")
+        out = {k: np.empty(shape, np.float64) for k in keys}
         check(lib().tr_batch_download(self._h, int(mid), *(_ptr(out[k], C.c_double) for k in ("h", "qx", "qy")),
-                                      _ptr(None, C.c_double)))
+                                      _ptr(out.get("maxh"), C.c_double)))
         return out
 
     @property
@@ -539,8 +598,9 @@ class Batch:
 class Triton:
     """Mirror of Triton::triton<T> (src/triton.h:32-327) for one .cfg:
     initialize() loads the inputs and uploads them, simulate() runs to
-    sim_duration and, like the reference, writes H/QX/QY rasters at each print
-    when an output folder is given."""
+    sim_duration and, like the reference, writes into an output folder when
+    one is given: rasters (and MH) and the hotstart .cfg at each print,
+    observation time series (triton_amd.output)."""
 
     def __init__(self, cfg_path, base_dir=None, form: int = FORM_F64_FUSED):
         self.cfg_path = cfg_path
@@ -553,16 +613,14 @@ class Triton:
         self.case = Case.load(self.cfg_path, self.base_dir)
         self.sim = Simulation(self.case, self.form)
 
-    def simulate(self, output_dir=None, pattern="%s/%s/%s_%02d_%02d") -> dict:
+    def simulate(self, output_dir=None) -> dict:
+        """Run to sim_duration; with an output_dir write what the reference
+        writes there (the cid file goes to base_dir, the reference's working
+        directory, or the current directory)."""
         if self.sim is None:
             raise TritonError(5, "simulate() before initialize()")
-        s = self.case.scalars
-        while True:
-            st = self.sim.run(t_stop=min(s["sim_duration"], s["print_interval"] * (self.sim_print_id() + 1)))
-            if output_dir is not None:
-                self.write_output(output_dir, st["print_id"], pattern)
-            if st["finished"]:
-                return st
+        from .output import run_with_outputs
+        return run_with_outputs(self.sim, self.case, output_dir, cid_dir=self.base_dir)
 
     def sim_print_id(self) -> int:
         return int(self.sim.sync()["print_id"])
@@ -574,8 +632,9 @@ class Triton:
 def write_rasters(state: dict, case: Case, output_dir, print_id: int, pattern=None) -> list:
     """Write a print's rasters as the reference's output::write_output does
     (src/output.h:512-709): the .cfg's print_option picks H ('h'), QX ('u')
-    and QY ('v'); output_format ASC writes the ASCII raster and its BIN twin
-    (the hotstart input), BIN only the binary one; file names follow
+    and QY ('v'), max_value_print_option 'h' adds MH (the running maximum
+    depth); output_format ASC writes the ASCII raster and its BIN twin (the
+    hotstart input), BIN only the binary one; file names follow
     outfile_pattern % (folder, format dir, variable, print id, rank 0)."""
     cfg = getattr(case, "config", {}) or {}
     pattern = pattern or cfg.get("outfile_pattern") or "%s/%s/%s_%02d_%02d"
@@ -586,8 +645,13 @@ def write_rasters(state: dict, case: Case, output_dir, print_id: int, pattern=No
     if fmt != "BIN":
         writers.append(("asc", L.tr_write_output_ascii))
     written = []
-    for letter, name, key in (("h", "H", "h"), ("u", "QX", "qx"), ("v", "QY", "qy")):
-        if letter not in opt:
+    mopt = (cfg.get("max_value_print_option") or "").lower()
+    for letter, name, key in (("h", "H", "h"), ("u", "QX", "qx"), ("v", "QY", "qy"), ("mh", "MH", "maxh")):
+        if letter == "mh":
+            # running maximum depth (src/output.h:560-575, 615-627)
+            if "h" not in mopt or state.get("maxh") is None:
+                continue
+        elif letter not in opt:
             continue
         a = np.ascontiguousarray(state[key])
         for sub, fn in writers:

@@ -19,6 +19,7 @@
 #include "common.hpp"
 
 namespace tr {
+int copy_obs_f32(int size, const float* arr, double* obs, const int* index, hipStream_t s);
 namespace {
 
 #define RC(x)            \
@@ -123,6 +124,10 @@ struct tr_batch {
   double* d_part = nullptr;
   double* d_sum = nullptr;
   double* h_sum = nullptr;
+  // observation sampling scratch (grown on demand)
+  int32_t* d_obs_idx = nullptr;
+  double* d_obs_val = nullptr;
+  int obs_cap = 0;
 
   ~tr_batch() {
     if (stream) hipStreamSynchronize(stream);
@@ -136,6 +141,8 @@ struct tr_batch {
     hipFree(d_members);
     hipFree(d_part);
     hipFree(d_sum);
+    hipFree(d_obs_idx);
+    hipFree(d_obs_val);
     if (h_ctrl) hipHostFree(h_ctrl);
     if (h_sum) hipHostFree(h_sum);
     if (stream) hipStreamDestroy(stream);
@@ -581,6 +588,11 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   P.t_stop = INFINITY;
   P.step_stop = INT64_MAX;
   P.halt_on_print = 0;
+  P.time_series = v.time_series_flag;
+  P.print_observation = v.print_observation;
+  P.halt_on_obs = 0;
+  P.it_print = v.it_print;
+  P.it_count0 = v.it_count;
   mm.interior = int64_t(rows - 2) * (cols - 2);
   mm.sim_duration = v.sim_duration;
   if (rc) {
@@ -598,6 +610,7 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   c0.print_id = v.checkpoint_id;
   if (c0.print_id == 0 && c0.simtime > 0.0) c0.print_id = int(c0.simtime / v.print_interval);
   c0.roff_row = v.roff_row0;
+  c0.last_obs = v.sim_start;  // src/triton_simulate.h:47
   cp.ctrl[0].dtmin_bits = ~0ull;
   cp.cursor = 1;
   RC(put_ctrl(b, id));
@@ -785,6 +798,10 @@ int tr_batch_status(tr_batch* b, int32_t id, tr_sim_status* st) {
   st->print_id = c.print_id;
   st->finished = c.finished;
   st->volume_in = c.vol_in;
+  st->observed = c.observed;
+  st->last_obs = c.last_obs;
+  st->average_dt = c.dt_acc;
+  st->it_count = mm->inst.it_count0 + c.step;
   const int cur = b->h_ctrl[id].cursor & 1;
   hipLaunchKernelGGL(k_volume_f32, dim3(kVolBlocks), dim3(256), 0, b->stream, mm->inst.h[cur],
                      mm->inst.rows, mm->inst.cols, mm->inst.dx * mm->inst.dx, b->d_part);
@@ -820,6 +837,10 @@ int tr_batch_poll(tr_batch* b, int32_t id, tr_sim_status* st) {
   st->finished = c.finished;
   st->volume = NAN;
   st->volume_in = c.vol_in;
+  st->observed = c.observed;
+  st->last_obs = c.last_obs;
+  st->average_dt = c.dt_acc;
+  st->it_count = mm->inst.it_count0 + c.step;
   return TR_OK;
 }
 
@@ -841,6 +862,56 @@ int tr_batch_download(tr_batch* b, int32_t id, double* h, double* qx, double* qy
   RC(get(mm->inst.qx[cur], qx));
   RC(get(mm->inst.qy[cur], qy));
   RC(get(mm->inst.maxh, maxh));
+  return TR_OK;
+}
+
+int tr_batch_sample(tr_batch* b, int32_t id, int32_t n, const int32_t* pos, double* h, double* qx,
+                    double* qy) {
+  Member* mm = member(b, id);
+  if (!mm || n < 0 || (n > 0 && !pos)) {
+    set_error("tr_batch_sample: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (n == 0) return TR_OK;
+  const int64_t cells = int64_t(mm->inst.rows) * mm->inst.cols;
+  for (int32_t i = 0; i < n; i++)
+    if (pos[i] < 0 || pos[i] >= cells) {
+      set_error("tr_batch_sample: cell index out of range");
+      return TR_ERR_INVALID;
+    }
+  RC(read_ctrls(b));
+  const int cur = b->h_ctrl[id].cursor & 1;
+  if (n > b->obs_cap) {
+    hipFree(b->d_obs_idx);
+    hipFree(b->d_obs_val);
+    b->d_obs_idx = nullptr;
+    b->d_obs_val = nullptr;
+    b->obs_cap = 0;
+    TR_HIP(hipMalloc(&b->d_obs_idx, sizeof(int32_t) * size_t(n)));
+    TR_HIP(hipMalloc(&b->d_obs_val, sizeof(double) * 3 * size_t(n)));
+    b->obs_cap = n;
+  }
+  TR_HIP(hipMemcpyAsync(b->d_obs_idx, pos, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice,
+                        b->stream));
+  const float* src[3] = {mm->inst.h[cur], mm->inst.qx[cur], mm->inst.qy[cur]};
+  double* dst[3] = {h, qx, qy};
+  for (int k = 0; k < 3; k++) {
+    if (!dst[k]) continue;
+    RC(copy_obs_f32(n, src[k], b->d_obs_val + size_t(k) * n, b->d_obs_idx, b->stream));
+    TR_HIP(hipMemcpyAsync(dst[k], b->d_obs_val + size_t(k) * n, sizeof(double) * size_t(n),
+                          hipMemcpyDeviceToHost, b->stream));
+  }
+  TR_HIP(hipStreamSynchronize(b->stream));
+  return TR_OK;
+}
+
+int tr_batch_set_obs_halt(tr_batch* b, int32_t id, int32_t on) {
+  Member* mm = member(b, id);
+  if (!mm) return TR_ERR_INVALID;
+  TR_HIP(hipStreamSynchronize(b->stream));
+  mm->inst.halt_on_obs = on ? 1 : 0;
+  RC(put_inst(b, id));
+  TR_HIP(hipStreamSynchronize(b->stream));
   return TR_OK;
 }
 

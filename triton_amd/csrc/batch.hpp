@@ -47,6 +47,11 @@ struct InstF32 {
   double sim_duration, print_interval, init_dt, t_stop;
   long long step_stop;
   int halt_on_print;
+  int time_series;          // observation points are sampled
+  double print_observation;
+  int halt_on_obs;          // stop after a step that completes an observation interval
+  int it_print;             // the reference's log cadence (the dt sum resets there)
+  long long it_count0;      // iterations before the start (hotstart)
 };
 
 // A piece is a run of rows of one column strip of one member; waves take

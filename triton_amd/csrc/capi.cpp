@@ -96,6 +96,9 @@ int tr_case_view_get(tr_case* cp, tr_case_view* v) {
   v->time_series_flag = c.cfg.time_series_flag;
   v->checkpoint_id = c.cfg.checkpoint_id;
   v->it_print = c.cfg.it_print;
+  v->it_count = c.cfg.it_count;
+  v->num_obs = c.cfg.time_series_flag ? int32_t(c.obs_pos.size()) : 0;
+  v->obs_pos = v->num_obs > 0 ? c.obs_pos.data() : nullptr;
   return TR_OK;
 }
 

@@ -481,11 +481,15 @@ void pad_and_finalize(Case& c, const std::vector<double>& dem_in,
     }
     if (c.hyg_nrows < 2) throw InputError("hydrograph needs at least two rows");
   }
-  // observation cells (src/triton_init.h:834-909; out-of-range only warns there)
+  // observation cells (src/triton_init.h:834-909).  The reference only
+  // prints an error for a point outside the DEM and then reads outside its
+  // arrays; here such a point is an input error.
   c.obs_pos.clear();
   for (size_t i = 0; i < cfg.obs_x.size(); i++) {
     int col = calc_col(cfg.obs_x[i], c.dem_hdr.xll, c.dem_hdr.cellsize);
     int row = calc_row(cfg.obs_y[i], c.dem_hdr.yll, c.dem_hdr.cellsize, nr);
+    if (cfg.time_series_flag && (col < 0 || col >= nc || row < 0 || row >= nr))
+      throw InputError("observation " + std::to_string(i + 1) + " is out of bounds");
     c.obs_pos.push_back((row + 1) * cols + (col + 1));
   }
 

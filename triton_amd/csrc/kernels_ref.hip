@@ -278,7 +278,34 @@ __global__ void k_wet_dry(int size, int nrows, int ncols, const double* __restri
   if (maxh && hc > maxh[id]) maxh[id] = hc;
 }
 
+// Observation points (src/kernels.h:521-535): obs_arr[i] = arr[index[i]].
+__global__ void k_copy_obs(int size, const double* __restrict__ arr, double* __restrict__ obs,
+                           const int* __restrict__ index) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < size) obs[id] = arr[index[id]];
+}
+
+__global__ void k_copy_obs_f32(int size, const float* __restrict__ arr, double* __restrict__ obs,
+                               const int* __restrict__ index) {
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < size) obs[id] = double(arr[index[id]]);
+}
+
 }  // namespace
+
+// Gathers of observation cells shared by the simulation and batch drivers.
+int copy_obs_f64(int size, const double* arr, double* obs, const int* index, hipStream_t s) {
+  if (size <= 0) return TR_OK;
+  hipLaunchKernelGGL(k_copy_obs, dim3(grid_for(size)), dim3(kBlock), 0, s, size, arr, obs, index);
+  return launch_check("copy_observation_points");
+}
+
+int copy_obs_f32(int size, const float* arr, double* obs, const int* index, hipStream_t s) {
+  if (size <= 0) return TR_OK;
+  hipLaunchKernelGGL(k_copy_obs_f32, dim3(grid_for(size)), dim3(kBlock), 0, s, size, arr, obs,
+                     index);
+  return launch_check("copy_observation_points_f32");
+}
 
 // Shared by the unfused simulation driver: min of `size` dt values into the
 // device slot (uint64 image), optionally also as a double.
@@ -447,6 +474,15 @@ int tr_wet_dry_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, const 
   hipLaunchKernelGGL(k_wet_dry, dim3(grid_for(size)), dim3(kBlock), 0, as_stream(stream), size,
                      nrows, ncols, h, qx, qy, dem, maxh, hextra, mpi_tasks);
   return launch_check("wet_dry");
+}
+
+int tr_copy_observation_points_f64(int32_t size, const double* arr, double* obs_arr,
+                                   const int32_t* relative_index, void* stream) {
+  if (size < 0 || (size > 0 && (!arr || !obs_arr || !relative_index))) {
+    set_error("tr_copy_observation_points_f64: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  return copy_obs_f64(size, arr, obs_arr, relative_index, as_stream(stream));
 }
 
 }  // extern "C"

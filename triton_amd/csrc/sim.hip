@@ -20,6 +20,7 @@
 
 namespace tr {
 int min_dt_to_slot(int size, const double* dtarr, unsigned long long* slot, hipStream_t s);
+int copy_obs_f64(int size, const double* arr, double* obs, const int* index, hipStream_t s);
 int launch_prime_f64(const StepParams<double>& p, hipStream_t s);
 int launch_step_f64(const StepParams<double>& p, hipStream_t s);
 int fused_grid_x(int cols);
@@ -57,6 +58,13 @@ struct tr_sim {
   int print_id = 0, roff_row = 0, idx_low = 0;
   int64_t steps = 0;
   double vol_in = 0;
+  double last_obs = 0, dt_acc = 0;
+  int observed = 0;
+  // observation sampling: halt after observation steps; gather scratch
+  int halt_on_obs = 0;
+  int32_t* d_obs_idx = nullptr;
+  double* d_obs_val = nullptr;
+  int obs_cap = 0;
   // fused control
   StepCtrl* ctrl = nullptr;
   StepCtrl* ctrl_host = nullptr;  // pinned mirror
@@ -75,6 +83,8 @@ struct tr_sim {
     if (batch) tr_batch_destroy(batch);
     if (stream) hipStreamSynchronize(stream);
     for (void* p : allocs) hipFree(p);
+    if (d_obs_idx) hipFree(d_obs_idx);
+    if (d_obs_val) hipFree(d_obs_val);
     if (ctrl_host) hipHostFree(ctrl_host);
     if (h_pinned) hipHostFree(h_pinned);
     if (stream) hipStreamDestroy(stream);
@@ -251,6 +261,14 @@ int step_unfused(tr_sim* s) {
   s->simtime += dt;
   s->last_dt = dt;
   s->steps++;
+  // the dt sum and the observation test of src/triton_simulate.h:66-81
+  s->dt_acc += dt;
+  if (v.it_print > 0 && (int64_t(v.it_count) + s->steps) % v.it_print == 0) s->dt_acc = 0.0;
+  s->observed = 0;
+  if (v.time_series_flag && s->simtime - s->last_obs >= v.print_observation) {
+    s->last_obs += v.print_observation;
+    s->observed = 1;
+  }
   if (s->simtime >= v.print_interval * (s->print_id + 1)) s->print_id++;
   return TR_OK;
 }
@@ -301,6 +319,11 @@ StepParams<double> fused_params(tr_sim* s, int64_t launch_index) {
   p.t_stop = INFINITY;
   p.step_stop = INT64_MAX;
   p.halt_on_print = 0;
+  p.time_series = v.time_series_flag;
+  p.print_observation = v.print_observation;
+  p.halt_on_obs = s->halt_on_obs;
+  p.it_print = v.it_print;
+  p.it_count0 = v.it_count;
   return p;
 }
 
@@ -335,6 +358,9 @@ int prime_fused(tr_sim* s) {
   c.idx_high = 0;
   c.vol_in = 0.0;
   c.step_vol_in = 0.0;
+  c.last_obs = v.sim_start;  // src/triton_simulate.h:47
+  c.dt_acc = 0.0;
+  c.observed = 0;
   RC(write_ctrl(s, 1, c));
   StepParams<double> p = fused_params(s, 0);
   p.cur = 1;  // the prime reduces into slot 1 and prepares slot 0 for step 0
@@ -417,6 +443,9 @@ int fill_status(tr_sim* s, tr_sim_status* st) {
     st->print_id = s->print_id;
     st->finished = !(s->simtime < s->v.sim_duration);
     st->volume_in = s->vol_in;
+    st->observed = s->observed;
+    st->last_obs = s->last_obs;
+    st->average_dt = s->dt_acc;
   } else {
     StepCtrl c;
     RC(realign_fused(s, c));
@@ -428,8 +457,12 @@ int fill_status(tr_sim* s, tr_sim_status* st) {
     st->finished = c.finished;
     st->volume_in = c.vol_in;
     st->dt = c.dt;
+    st->observed = c.observed;
+    st->last_obs = c.last_obs;
+    st->average_dt = c.dt_acc;
     h = s->h[c.step & 1];
   }
+  st->it_count = int64_t(s->v.it_count) + st->steps;
   return volume_f64(s, h, &st->volume);
 }
 
@@ -587,6 +620,7 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
     s->print_id = v.checkpoint_id;
     if (s->print_id == 0 && s->simtime > 0.0) s->print_id = int(s->simtime / v.print_interval);
     s->roff_row = v.roff_row0;
+    s->last_obs = v.sim_start;  // src/triton_simulate.h:47
   }
   if (!rc && form != TR_FORM_F64_UNFUSED) {
     rc = s->alloc(s->ctrl, 2);
@@ -622,6 +656,7 @@ int tr_sim_run(tr_sim* s, double t_stop, int64_t max_steps, tr_sim_status* st) {
     while (s->simtime < stop && (max_steps < 0 || done < max_steps)) {
       RC(step_unfused(s));
       done++;
+      if (s->halt_on_obs && s->observed) break;
     }
   } else {
     RC(run_fused(s, stop, max_steps));
@@ -688,6 +723,59 @@ int tr_sim_download(tr_sim* s, double* h, double* qx, double* qy, double* maxh) 
   if (qy) TR_HIP(hipMemcpyAsync(qy, s->qy[b], n, hipMemcpyDeviceToHost, s->stream));
   if (maxh && s->maxh) TR_HIP(hipMemcpyAsync(maxh, s->maxh, n, hipMemcpyDeviceToHost, s->stream));
   TR_HIP(hipStreamSynchronize(s->stream));
+  return TR_OK;
+}
+
+int tr_sim_sample(tr_sim* s, int32_t n, const int32_t* pos, double* h, double* qx, double* qy) {
+  if (!s || n < 0 || (n > 0 && !pos)) {
+    set_error("tr_sim_sample: bad arguments");
+    return TR_ERR_INVALID;
+  }
+  if (s->batch) return tr_batch_sample(s->batch, s->member, n, pos, h, qx, qy);
+  if (n == 0) return TR_OK;
+  const int64_t cells = int64_t(s->v.rows) * s->v.cols;
+  for (int32_t i = 0; i < n; i++)
+    if (pos[i] < 0 || pos[i] >= cells) {
+      set_error("tr_sim_sample: cell index out of range");
+      return TR_ERR_INVALID;
+    }
+  int b = 0;
+  if (s->form != TR_FORM_F64_UNFUSED) {
+    StepCtrl c;
+    RC(realign_fused(s, c));
+    b = int(c.step & 1);
+  }
+  if (n > s->obs_cap) {
+    if (s->d_obs_idx) hipFree(s->d_obs_idx);
+    if (s->d_obs_val) hipFree(s->d_obs_val);
+    s->d_obs_idx = nullptr;
+    s->d_obs_val = nullptr;
+    s->obs_cap = 0;
+    TR_HIP(hipMalloc(&s->d_obs_idx, sizeof(int32_t) * size_t(n)));
+    TR_HIP(hipMalloc(&s->d_obs_val, sizeof(double) * 3 * size_t(n)));
+    s->obs_cap = n;
+  }
+  TR_HIP(hipMemcpyAsync(s->d_obs_idx, pos, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice,
+                        s->stream));
+  const double* src[3] = {s->h[b], s->qx[b], s->qy[b]};
+  double* dst[3] = {h, qx, qy};
+  for (int k = 0; k < 3; k++) {
+    if (!dst[k]) continue;
+    RC(copy_obs_f64(n, src[k], s->d_obs_val + size_t(k) * n, s->d_obs_idx, s->stream));
+    TR_HIP(hipMemcpyAsync(dst[k], s->d_obs_val + size_t(k) * n, sizeof(double) * size_t(n),
+                          hipMemcpyDeviceToHost, s->stream));
+  }
+  TR_HIP(hipStreamSynchronize(s->stream));
+  return TR_OK;
+}
+
+int tr_sim_set_obs_halt(tr_sim* s, int32_t on) {
+  if (!s) {
+    set_error("tr_sim_set_obs_halt: NULL sim");
+    return TR_ERR_INVALID;
+  }
+  if (s->batch) return tr_batch_set_obs_halt(s->batch, s->member, on);
+  s->halt_on_obs = on ? 1 : 0;
   return TR_OK;
 }
 

@@ -24,10 +24,11 @@ struct alignas(128) StepCtrl {
   int roff_row;                   // runoff row applied in this step
   int idx_low, idx_high;          // hydrograph rows bracketing simtime
   int finished;                   // simtime >= sim_duration
-  int pad0;
+  int observed;                   // the step just done completed an observation interval
   double vol_in;                  // FP64 rain + inflow volume before this step
   double step_vol_in;             // volume this step adds
-  double pad1[2];
+  double last_obs;                // time of the last observation interval boundary
+  double dt_acc;                  // sum of dt since the last it_print boundary
 };
 
 constexpr int kMaxSrcFused = 256;  // point sources handled by the fused step
@@ -70,6 +71,11 @@ struct StepParams {
   double sim_duration, print_interval, init_dt, t_stop;
   long long step_stop;
   int halt_on_print;
+  int time_series;          // observation points are sampled
+  double print_observation;
+  int halt_on_obs;          // stop after a step that completes an observation interval
+  int it_print;             // the reference's log cadence (the dt sum resets there)
+  long long it_count0;      // iterations before the start (hotstart)
 };
 
 }  // namespace tr

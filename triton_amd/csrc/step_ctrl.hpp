@@ -74,9 +74,25 @@ __device__ void prepare_next(const P& p, const StepCtrl& pv, StepCtrl& nx,
       src_vol += q * dt;
     }
   }
+  // observation points (src/triton_simulate.h:78-81): sampled after a step
+  // once simtime has moved print_observation past the last boundary, which
+  // then advances by one interval
+  double last_obs = pv.last_obs;
+  bool observed = false;
+  if (after_step && p.time_series && simtime - last_obs >= p.print_observation) {
+    last_obs += p.print_observation;
+    observed = true;
+  }
+  // the dt sum the reference writes as time_step= (src/triton_simulate.h:
+  // 66-75): accumulated every step, cleared on each it_print boundary
+  double dt_acc = pv.dt_acc;
+  if (after_step) {
+    dt_acc += pv.dt;
+    if (p.it_print > 0 && (p.it_count0 + step) % p.it_print == 0) dt_acc = 0.0;
+  }
   bool finished = !(simtime < p.sim_duration);
   bool halt = finished || !(simtime < p.t_stop) || step >= p.step_stop ||
-              (p.halt_on_print && printed);
+              (p.halt_on_print && printed) || (p.halt_on_obs && observed);
 
   nx.simtime = simtime;
   nx.dt = dt;
@@ -90,6 +106,9 @@ __device__ void prepare_next(const P& p, const StepCtrl& pv, StepCtrl& nx,
   nx.finished = finished ? 1 : 0;
   nx.vol_in = after_step ? pv.vol_in + pv.step_vol_in : pv.vol_in;
   nx.step_vol_in = rain_vol + src_vol;
+  nx.observed = observed ? 1 : 0;
+  nx.last_obs = last_obs;
+  nx.dt_acc = dt_acc;
 }
 
 // A skipped (halted) step carries its control block and source increments

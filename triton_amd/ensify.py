@@ -6,10 +6,11 @@ members as [rank_beg, rank_end, [arguments...], stdout_file, stderr_file]
 own MPI job with its .cfg as argument (src/Ensify.h:47-72, src/main.cpp:
 69-82).  Here the same file drives one batch: every member's .cfg is loaded
 with the reference's input formats, the members advance together in one
-launch per step (FP32 batched form) or one after another (FP64 forms), each
-member's rasters are written to its own output_folder at its print times,
-and its log goes to its stdout file.  MPI ranks have no meaning on one GPU;
-they are kept for the record.
+launch per step (FP32 batched form) or one after another (FP64 forms), and
+each member's output folder gets what the reference writes there
+(triton_amd.output: rasters, MH and the hotstart .cfg at its prints,
+observation time series); its log goes to its stdout file.  MPI ranks have
+no meaning on one GPU; they are kept for the record.
 """
 from __future__ import annotations
 
@@ -19,7 +20,8 @@ from pathlib import Path
 
 import yaml
 
-from . import FORM_F32_FUSED, FORM_F64_FUSED, FORM_F64_UNFUSED, Batch, Case, Simulation, write_rasters
+from . import FORM_F32_FUSED, FORM_F64_FUSED, FORM_F64_UNFUSED, Batch, Case, Simulation
+from .output import TimeSeries, run_with_outputs, write_print
 
 
 @dataclass
@@ -93,40 +95,43 @@ def _folder(m: Ensemble, base: Path) -> Path:
 
 
 def _run_one(m: Ensemble, base: Path, form: int, write_outputs: bool) -> None:
-    s = m.case.scalars
     with Simulation(m.case, form) as sim:
-        pid = int(sim.sync()["print_id"])
-        while True:
-            st = sim.run(t_stop=min(s["sim_duration"], s["print_interval"] * (pid + 1)))
-            if write_outputs and st["print_id"] > pid:
-                m.outputs += write_rasters(sim.state(), m.case, _folder(m, base), st["print_id"])
-            pid = st["print_id"]
-            if st["finished"]:
-                m.status = st
-                return
+        folder = _folder(m, base) if write_outputs else None
+        m.status = run_with_outputs(sim, m.case, folder, cid_dir=base, written=m.outputs)
 
 
 def _run_batched(members, base: Path, write_outputs: bool, chunk_steps: int) -> None:
     with Batch() as b:
         ids = [b.add(m.case) for m in members]
-        written = {}
-        for mid in ids:
-            if write_outputs:
-                b.set_print_halt(mid, True)
+        written, series, obs_step = {}, {}, {}
+        for mid, m in zip(ids, members):
+            if not write_outputs:
+                continue
+            b.set_print_halt(mid, True)
+            if int(m.case.scalars.get("time_series_flag", 0)) and m.case.obs_pos.shape[0] > 0:
+                series[mid] = TimeSeries(m.case, _folder(m, base))
+                series[mid].append(m.case.scalars.get("sim_start", 0.0), b.sample(mid))
+                b.set_obs_halt(mid, True)
         b.arm()
         b.sync()
         for mid in ids:
-            written[mid] = b.poll(mid)["print_id"]
+            st = b.poll(mid)
+            written[mid], obs_step[mid] = st["print_id"], st["steps"]
         active = len(ids)
         while active:
             b.enqueue(chunk_steps)
             b.sync()
             for mid, m in zip(ids, members):
                 st = b.poll(mid)
+                if mid in series and st["observed"] and st["steps"] != obs_step[mid]:
+                    series[mid].append(st["simtime"], b.sample(mid))
+                    obs_step[mid] = st["steps"]
                 if write_outputs and st["print_id"] > written[mid]:
-                    m.outputs += write_rasters(b.state(mid), m.case, _folder(m, base), st["print_id"])
+                    m.outputs += write_print(b.state(mid), m.case, _folder(m, base), st, cid_dir=base)
                     written[mid] = st["print_id"]
-            b.arm()  # releases members halted at a print; finished ones stay out
+            b.arm()  # releases members halted at a print or observation; finished ones stay out
             active = b.sync()
         for mid, m in zip(ids, members):
             m.status = b.status(mid)
+            if mid in series:
+                series[mid].append(m.status["simtime"], b.sample(mid))
