@@ -60,12 +60,14 @@ __device__ __forceinline__ TR_GLOBAL T* gptr(T* p) {
 // Neighbour-lane moves across the whole 64-lane wave with DPP (wave_shl:1 /
 // wave_shr:1, GFX9-family controls): a VALU move instead of an LDS
 // permute.  Lane i receives lane i+1 (from_right) or lane i-1 (from_left);
-// the edge lane receives 0.  Callers use them in wave-uniform control flow.
+// the edge lane receives 0, written by the hardware (bound_ctrl), so the
+// destination needs no initial value and each shift is one instruction.
+// Callers use them in wave-uniform control flow.
 __device__ __forceinline__ int dpp_from_right(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int dpp_from_left(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+  return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float from_right(float x) {
   return __int_as_float(dpp_from_right(__float_as_int(x)));
