@@ -138,3 +138,134 @@ def test_allatoona_unfused_matches_reference(gpu, ref_tree):
         ref = load_asc(ref_tree / "test/reference/allatoona" / f"{fld}_02_00.out")
         got = as_printed(s[k])
         assert np.array_equal(got, ref), (fld, np.abs(got - ref).max(), int((got != ref).sum()))
+
+
+def test_allatoona_fused_forms(gpu, ref_tree):
+    """The same fixture in the fused forms: external boundaries inside the
+    row loop.  FP64 fused equals the unfused form bit for bit (both use the
+    device cbrt); FP32 stays within its stated tolerance of the fixture."""
+    case = ta.Case.load(ref_tree / "test/allatoona.cfg", base_dir=ref_tree / "test")
+    with ta.Simulation(case, ta.FORM_F64_UNFUSED) as sim:
+        su = sim.run()
+        u = sim.state()
+    with ta.Simulation(case, ta.FORM_F64_FUSED) as sim:
+        sf = sim.run()
+        f = sim.state()
+    assert sf["steps"] == su["steps"] and sf["simtime"] == su["simtime"]
+    for k in ("h", "qx", "qy"):
+        assert np.array_equal(f[k], u[k]), (k, np.abs(f[k] - u[k]).max())
+    with ta.Simulation(case, ta.FORM_F32_FUSED) as sim:
+        s32 = sim.run()
+        g = sim.state()
+    assert s32["finished"] and s32["print_id"] == 2
+    ref = load_asc(ref_tree / "test/reference/allatoona" / "H_02_00.out")
+    assert np.abs(as_printed(g["h"]) - ref).max() <= 1e-2
+
+
+def with_extbc(case, entries):
+    """Attach external boundary entries to a synthetic case: (row, col) of an
+    interior cell (unpadded), type, and its series ((t, level) rows for type
+    1, one parameter for types 2 and 3, nothing for type 0)."""
+    cols = case.cols
+    idx, typ, start, nrows, v1, v2 = [], [], [], [], [], []
+    for (r, c), t, series in entries:
+        idx.append((r + 1) * cols + c + 1)
+        typ.append(t)
+        start.append(len(v1))
+        rows = series if t == 1 else ([(series, 0.0)] if t in (2, 3) else [(0.0, 0.0)])
+        nrows.append(len(rows))
+        for a, b in rows:
+            v1.append(a)
+            v2.append(b)
+    case.arrays.update(bc_index=np.array(idx, np.int32), bc_type=np.array(typ, np.int32),
+                       bc_start=np.array(start, np.int32), bc_nrows=np.array(nrows, np.int32),
+                       bc_var1=np.array(v1), bc_var2=np.array(v2))
+    return case
+
+
+def extbc_entries(nr, nc, cbrt_free=False):
+    """Every edge and corner, each type; a level series that rises and falls.
+    cbrt_free: no normal-slope entries (type 2 takes a cube root)."""
+    lvl = [(0.0, 0.2), (3.0, 0.9), (6.0, 0.4), (60.0, 0.4)]
+    t2 = 3 if cbrt_free else 2
+    e = []
+    for c in range(0, nc, 3):
+        e.append(((0, c), (0, 1, t2, 3)[c % 4], lvl if c % 4 == 1 else (0.8 if c % 4 == 3 else 0.002)))
+    for c in range(1, nc, 3):
+        e.append(((nr - 1, c), (1, 3, 0, t2)[c % 4], lvl if c % 4 == 0 else (0.6 if c % 4 == 1 else 0.001)))
+    for r in range(2, nr - 2, 2):
+        e.append(((r, 0), (3, 1)[r % 4 == 0], 0.7 if r % 4 else lvl))
+        e.append(((r, nc - 1), (t2, 0)[r % 4 == 0], 0.003))
+    taken = {rc for rc, _, _ in e}
+    for rc in ((0, nc - 1), (nr - 1, 0), (nr - 1, nc - 1)):
+        if rc not in taken:
+            e.append((rc, 3, 0.5))
+    return e
+
+
+@pytest.mark.parametrize("friction", [False, True])
+@pytest.mark.parametrize("open_bc", [0, 1])
+@pytest.mark.parametrize("shape", [(23, 67), (61, 61), (130, 9)])
+def test_extbc_fused_equals_unfused_and_oracle(gpu, shape, open_bc, friction):
+    """All four boundary types on every edge and the corners, walls and open
+    boundaries.  The fused FP64 pass equals the unfused drop-in kernels bit
+    for bit.  Without a cube root (no friction, no normal-slope entries) both
+    equal the oracle bit for bit; with one the device's and glibc's cbrt
+    differ by an ulp on some inputs, and thin films carry that further
+    (stated tolerance).  FP32 within 1e-2 m."""
+    nr, nc = shape
+    case = with_extbc(synthetic(nr, nc, seed=nr + nc, open_bc=open_bc, dur=8.0, sources=False,
+                                friction=friction), extbc_entries(nr, nc, cbrt_free=not friction))
+    orc = Oracle.from_case(case)
+    ro = orc.run()
+    out = {}
+    for form in (ta.FORM_F64_UNFUSED, ta.FORM_F64_FUSED, ta.FORM_F32_FUSED):
+        with ta.Simulation(case, form) as sim:
+            out[form] = (sim.run(), sim.state())
+    for k in ("h", "qx", "qy"):
+        assert np.array_equal(out[ta.FORM_F64_FUSED][1][k], out[ta.FORM_F64_UNFUSED][1][k]), k
+    for form in (ta.FORM_F64_UNFUSED, ta.FORM_F64_FUSED):
+        st, s = out[form]
+        assert st["steps"] == ro["steps"], form
+        for k in ("h", "qx", "qy"):
+            d = np.abs(s[k] - orc.state[k]).max()
+            assert d <= (1e-3 if friction else 0.0), (form, k, d)
+    st32, s32 = out[ta.FORM_F32_FUSED]
+    assert st32["finished"]
+    assert np.abs(s32["h"] - orc.state["h"]).max() <= 1e-2
+
+
+def test_extbc_cell_listed_twice(gpu):
+    """A cell in two entries takes them in list order in every form (the
+    Allatoona file has such cells): a level (type 1) and then a Froude
+    velocity (type 3) from that level."""
+    case = with_extbc(synthetic(20, 20, seed=3, dur=4.0, sources=False),
+                      [((0, 5), 1, [(0.0, 0.9), (10.0, 1.2)]), ((0, 5), 3, 0.5), ((7, 0), 2, 0.01),
+                       ((7, 0), 1, [(0.0, 0.7), (10.0, 0.8)])])
+    orc = Oracle.from_case(case)
+    orc.run()
+    for form in (ta.FORM_F64_UNFUSED, ta.FORM_F64_FUSED):
+        with ta.Simulation(case, form) as sim:
+            sim.run()
+            s = sim.state()
+        for k in ("h", "qx", "qy"):
+            assert np.abs(s[k] - orc.state[k]).max() <= 1e-9, (form, k)
+
+
+def test_member_bits_do_not_depend_on_a_segment_member_in_the_batch(gpu):
+    """A batch with an external-boundary member runs the step variant that has
+    the segment code; a member without segments gives the same bits there as
+    alone (common variant)."""
+    plain = synthetic(45, 70, seed=11, dur=3.0)
+    seg = with_extbc(synthetic(30, 40, seed=12, dur=3.0, sources=False), extbc_entries(30, 40))
+    with ta.Batch() as b:
+        i = b.add(plain)
+        b.run()
+        alone = b.state(i)
+    with ta.Batch() as b:
+        i = b.add(plain)
+        b.add(seg)
+        b.run()
+        mixed = b.state(i)
+    for k in ("h", "qx", "qy"):
+        assert np.array_equal(alone[k], mixed[k]), k

@@ -110,6 +110,7 @@ struct tr_batch {
   int* d_first = nullptr;
   int* d_members = nullptr;    // [cap] active members, in partition order
   int nmembers = 0;
+  int any_bc = 0;              // some member in the partition has external boundaries
   int piece_cap = 0, first_cap = 0;
   int nwork = 0;               // waves per launch
   int npieces = 0;
@@ -236,8 +237,9 @@ int check_member_view(const tr_case_view* v) {
     set_error("tr_batch_add: dx, courant and print_interval must be positive");
     return TR_ERR_INVALID;
   }
-  if (v->num_extbc_cells > 0) {
-    set_error("tr_batch_add: external boundary segments run only in TR_FORM_F64_UNFUSED");
+  if (v->num_extbc_cells > 0 && (!v->bc_index || !v->bc_type || !v->bc_start || !v->bc_nrows ||
+                                  !v->bc_var1 || !v->bc_var2 || v->bc_nvars < 1)) {
+    set_error("tr_batch_add: external boundary segments need their index, type and series arrays");
     return TR_ERR_INVALID;
   }
   if (v->num_runoffs > 0 && (v->roff_nrows < 2 || !v->roff_time || !v->roff_rate)) {
@@ -282,6 +284,7 @@ int64_t estimate_bytes(const tr_case_view& v, bool fric, bool zones) {
   if (zones) b += n * 4;
   b += int64_t(v.num_runoffs) * (v.roff_nrows + 3) * 8 + int64_t(v.roff_nrows) * 8;
   b += int64_t(v.num_src) * (v.hyg_nrows + 6) * 8 + int64_t(v.hyg_nrows) * 8;
+  if (v.num_extbc_cells > 0) b += n * 4 + int64_t(v.num_extbc_cells) * 28 + int64_t(v.bc_nvars) * 16;
   return b + 4096;
 }
 
@@ -305,7 +308,11 @@ int rebuild_work(tr_batch* b) {
     return b->m[size_t(x)].interior > b->m[size_t(y)].interior;
   });
   int64_t total = 0;
-  for (int i : order) total += int64_t(b->m[size_t(i)].inst.nbx) * b->m[size_t(i)].inst.rows;
+  b->any_bc = 0;
+  for (int i : order) {
+    total += int64_t(b->m[size_t(i)].inst.nbx) * b->m[size_t(i)].inst.rows;
+    if (b->m[size_t(i)].inst.bc.n > 0) b->any_bc = 1;
+  }
   std::vector<Piece> pieces;
   std::vector<int> first;
   int nw = 0;
@@ -579,6 +586,28 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
     P.hyg_time = ht;
     P.hyg_val = hv;
   }
+  P.bc = ExtBC{};
+  if (!rc && v.num_extbc_cells > 0) {
+    ExtBCHost bch;
+    const std::string err = build_extbc(rows, cols, v.num_extbc_cells, v.bc_index, v.bc_type,
+                                        v.bc_start, v.bc_nrows, v.dem, v.nman2, bch);
+    if (!err.empty()) {
+      set_error("tr_batch_add: " + err);
+      rc = TR_ERR_INVALID;
+    }
+    int *map = nullptr, *nx = nullptr, *ty = nullptr, *st = nullptr, *nv = nullptr;
+    double *v1 = nullptr, *v2 = nullptr, *zc = nullptr, *n2c = nullptr;
+    if (!rc) rc = m_upload(mm, map, bch.map.data(), bch.map.size(), s);
+    if (!rc) rc = m_upload(mm, nx, bch.next.data(), bch.next.size(), s);
+    if (!rc) rc = m_upload(mm, ty, bch.type.data(), bch.type.size(), s);
+    if (!rc) rc = m_upload(mm, st, bch.start.data(), bch.start.size(), s);
+    if (!rc) rc = m_upload(mm, nv, bch.nvars.data(), bch.nvars.size(), s);
+    if (!rc) rc = m_upload(mm, v1, v.bc_var1, size_t(v.bc_nvars), s);
+    if (!rc) rc = m_upload(mm, v2, v.bc_var2, size_t(v.bc_nvars), s);
+    if (!rc) rc = m_upload(mm, zc, bch.zc.data(), bch.zc.size(), s);
+    if (!rc) rc = m_upload(mm, n2c, bch.n2c.data(), bch.n2c.size(), s);
+    if (!rc) P.bc = ExtBC{v.num_extbc_cells, map, nx, ty, st, nv, v1, v2, zc, n2c};
+  }
   P.open_bc = v.open_boundaries;
   P.ctrl = &b->d_ctrl[id].ctrl[0];
   P.cursor = &b->d_ctrl[id].cursor;
@@ -672,7 +701,8 @@ namespace {
 
 bool same_args(const BatchArgs& x, const BatchArgs& y) {
   return x.inst == y.inst && x.pieces == y.pieces && x.wave_first == y.wave_first &&
-         x.nwaves == y.nwaves && x.members == y.members && x.nmembers == y.nmembers;
+         x.nwaves == y.nwaves && x.members == y.members && x.nmembers == y.nmembers &&
+         x.any_bc == y.any_bc;
 }
 
 // Capture graph_steps batched steps (step + close kernels only: no memset
@@ -715,7 +745,7 @@ int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
     return TR_ERR_INVALID;
   }
   if (b->table_dirty) RC(rebuild_work(b));
-  BatchArgs a{b->d_inst, b->d_pieces, b->d_first, b->nwork, b->d_members, b->nmembers};
+  BatchArgs a{b->d_inst, b->d_pieces, b->d_first, b->nwork, b->d_members, b->nmembers, b->any_bc};
   int64_t k = 0;
   if (b->graph_steps > 0 && a.nwaves > 0 && nsteps >= b->graph_steps) {
     if (!b->gexec || b->gsteps_built != b->graph_steps || !same_args(a, b->gargs)) RC(build_graph(b, a));

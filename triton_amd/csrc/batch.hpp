@@ -52,6 +52,7 @@ struct InstF32 {
   int halt_on_obs;          // stop after a step that completes an observation interval
   int it_print;             // the reference's log cadence (the dt sum resets there)
   long long it_count0;      // iterations before the start (hotstart)
+  ExtBC bc;                 // external boundary segments (bc.n == 0: none)
 };
 
 // A piece is a run of rows of one column strip of one member; waves take
@@ -70,6 +71,7 @@ struct BatchArgs {
   int nwaves;
   const int* members;        // members in the partition (the close kernel's list)
   int nmembers;
+  int any_bc;                // some member has external boundary segments
 };
 
 int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s);

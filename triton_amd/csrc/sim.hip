@@ -48,6 +48,9 @@ struct tr_sim {
   double* src_add = nullptr;
   int32_t *bc_index = nullptr, *bc_type = nullptr, *bc_start = nullptr, *bc_nrows = nullptr;
   double *bc_v1 = nullptr, *bc_v2 = nullptr;
+  int32_t* bc_map = nullptr;                 // fused forms: edge cell -> first entry
+  int32_t* bc_next = nullptr;                // fused forms: next entry of the same cell
+  double *bc_zc = nullptr, *bc_n2c = nullptr;
   bool friction = false;
   // unfused scratch
   double *sqrth = nullptr, *dtarr = nullptr, *rhs[6] = {};
@@ -324,6 +327,19 @@ StepParams<double> fused_params(tr_sim* s, int64_t launch_index) {
   p.halt_on_obs = s->halt_on_obs;
   p.it_print = v.it_print;
   p.it_count0 = v.it_count;
+  p.bc = ExtBC{};
+  if (v.num_extbc_cells > 0) {
+    p.bc.n = v.num_extbc_cells;
+    p.bc.map = s->bc_map;
+    p.bc.next = s->bc_next;
+    p.bc.type = s->bc_type;
+    p.bc.start = s->bc_start;
+    p.bc.nvars = s->bc_nrows;
+    p.bc.v1 = s->bc_v1;
+    p.bc.v2 = s->bc_v2;
+    p.bc.zc = s->bc_zc;
+    p.bc.n2c = s->bc_n2c;
+  }
   return p;
 }
 
@@ -507,9 +523,15 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
     set_error("tr_sim_create: unknown form");
     return TR_ERR_INVALID;
   }
-  if (form != TR_FORM_F64_UNFUSED && view->num_extbc_cells > 0) {
-    set_error("tr_sim_create: external boundary segments run only in TR_FORM_F64_UNFUSED");
-    return TR_ERR_INVALID;
+  ExtBCHost bch;
+  if (form != TR_FORM_F64_UNFUSED) {
+    const std::string err = build_extbc(view->rows, view->cols, view->num_extbc_cells,
+                                        view->bc_index, view->bc_type, view->bc_start,
+                                        view->bc_nrows, view->dem, view->nman2, bch);
+    if (!err.empty()) {
+      set_error("tr_sim_create: " + err);
+      return TR_ERR_INVALID;
+    }
   }
   if (form != TR_FORM_F64_UNFUSED && view->num_src > kMaxSrcFused) {
     set_error("tr_sim_create: too many point sources for the fused step");
@@ -602,6 +624,12 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
     if (!rc) rc = s->upload(s->bc_nrows, v.bc_nrows, k);
     if (!rc) rc = s->upload(s->bc_v1, v.bc_var1, size_t(std::max(v.bc_nvars, 1)));
     if (!rc) rc = s->upload(s->bc_v2, v.bc_var2, size_t(std::max(v.bc_nvars, 1)));
+    if (!rc && form != TR_FORM_F64_UNFUSED) {  // the fused pass's edge map
+      rc = s->upload(s->bc_map, bch.map.data(), bch.map.size());
+      if (!rc) rc = s->upload(s->bc_next, bch.next.data(), bch.next.size());
+      if (!rc) rc = s->upload(s->bc_zc, bch.zc.data(), bch.zc.size());
+      if (!rc) rc = s->upload(s->bc_n2c, bch.n2c.data(), bch.n2c.size());
+    }
   }
   if (!rc) rc = s->alloc(s->vpart, kVolBlocks);
   if (!rc) rc = s->alloc(s->vsum, 1);

@@ -7,6 +7,8 @@
 
 #include <cstdint>
 
+#include "extbc.hpp"
+
 namespace tr {
 
 // Control block of one step.  Two slots ping-pong: step n reads slot n&1 and
@@ -76,6 +78,7 @@ struct StepParams {
   int halt_on_obs;          // stop after a step that completes an observation interval
   int it_print;             // the reference's log cadence (the dt sum resets there)
   long long it_count0;      // iterations before the start (hotstart)
+  ExtBC bc;                 // external boundary segments (bc.n == 0: none)
 };
 
 }  // namespace tr

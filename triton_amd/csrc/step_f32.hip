@@ -251,6 +251,7 @@ __device__ __forceinline__ Raw load_row(const Loc& L, int r, unsigned col4) {
 
 // One member's piece: rows [row_first, row_last] of column strip bx, with
 // the two-row halo recomputed at its ends.  Returns the piece's dt min bits.
+template <bool kBC>
 __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, int e_min, int e_max,
                                             int lane) {
   const StepCtrl& CC = P.ctrl[cur];
@@ -302,6 +303,8 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   // updated rows r-2 (A) and r-3 (B)
   float ah = 0.f, aqx = 0.f, aqy = 0.f, adzx = 0.f, adzy = 0.f;
   float bh = 0.f;
+  // external boundary values carried to the south ghost row
+  SouthCarry<float> sc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, false};
   float best = kMaxDtF * 2.f;
   int ks = 0;  // first source not above the row being updated
 
@@ -401,6 +404,10 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
       if (ru == 1) { ah = nh; aqx = nqx; aqy = nqy; }
       if (ru == R - 1) { nh = ah; nqx = aqx; nqy = aqy; }
     }
+    // external boundary segments after the open-boundary copies
+    if (kBC && P.bc.n > 0)
+      extbc_row(P.bc, ru, c, R, Cc, col_int, edge_we, open_bc, CC.simtime, dt64, nh, nqx, nqy, ah,
+                aqx, aqy, sc);
 
     // wet/dry, dt and store of row r-2
     const int e = r - 2;
@@ -451,6 +458,9 @@ __device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float best, int la
 // Batched step: wave w (placed XCD-contiguously) walks its pieces in order
 // and publishes one dt min per run of pieces of the same member.
 // 5 waves per SIMD: the step fits in 96 VGPRs without spills.
+// kBC: some member has external boundary segments (a variant of its own, so
+// the common one keeps its registers).
+template <bool kBC>
 __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const int w = xcd_slot(int(blockIdx.x), a.nwaves);
   const int lane = threadIdx.x;
@@ -470,7 +480,7 @@ __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
       cc = P.ctrl + cur;
       acc = kMaxDtF * 2.f;
     }
-    acc = fminf(acc, step_piece(P, cur, pc.bx, pc.row_first, pc.row_last, lane));
+    acc = fminf(acc, step_piece<kBC>(P, cur, pc.bx, pc.row_first, pc.row_last, lane));
   }
   if (cc) publish_min_f32(cc, acc, lane);
 }
@@ -517,7 +527,8 @@ int default_strip_rows(int rows, int cols) {
 
 int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s) {
   if (nwaves <= 0) return TR_OK;
-  hipLaunchKernelGGL(k_step_f32, dim3(unsigned(nwaves)), dim3(kLanes), 0, s, a);
+  if (a.any_bc) hipLaunchKernelGGL(k_step_f32<true>, dim3(unsigned(nwaves)), dim3(kLanes), 0, s, a);
+  else hipLaunchKernelGGL(k_step_f32<false>, dim3(unsigned(nwaves)), dim3(kLanes), 0, s, a);
   hipLaunchKernelGGL(k_close_f32, dim3(unsigned((a.nmembers + kLanes - 1) / kLanes)), dim3(kLanes), 0, s,
                      a.inst, a.members, 0, a.nmembers, 1);
   return launch_check("step_f32");

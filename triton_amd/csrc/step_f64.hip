@@ -48,12 +48,13 @@ __global__ __launch_bounds__(64) void k_close_f64(StepParams<double> p, int afte
   else close_one(p, p.cur, after_step != 0);
 }
 
-template <bool kOpen>
+template <bool kOpen, bool kBC>
 __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const int lane = threadIdx.x;
   const StepCtrl* cc = p.ctrl + p.cur;
   if (cc->halt) return;
   const double dt = cc->dt;
+  const double simtime = cc->simtime;
   const int roff_row = cc->roff_row;
   const double* __restrict__ src_add = p.src_add + p.cur * p.nsrc;
   const double dx = p.dx, hextra = p.hextra;
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const bool ghost_w = c == 0 && lane == 1;
   const bool ghost_e = c == C - 1 && lane >= 3 && lane <= kLanes - 2;
   const bool col_mine = out_lane || ghost_w || ghost_e;
+  const bool edge_we = bx == 0 || bx == p.nbx - 1;
 
   const int e_lo = 1 + p.strip_rows * by;
   const int e_hi = min(e_lo + p.strip_rows - 1, R - 2);
@@ -85,6 +87,8 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   // updated rows r-2 (A) and r-3 (B)
   double ah = 0.0, aqx = 0.0, aqy = 0.0, az = 0.0;
   double bh = 0.0, bz = 0.0;
+  // external boundary values carried to the south ghost row
+  SouthCarry<double> sc{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false};
   unsigned long long best = ~0ull;
 
   for (int r = e_min - 2; r <= e_max + 2; ++r) {
@@ -173,6 +177,10 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
       if (ru == 1) { ah = nh; aqx = nqx; aqy = nqy; }          // row 0 <- row 1
       if (ru == R - 1) { nh = ah; nqx = aqx; nqy = aqy; }      // row R-1 <- row R-2
     }
+    // external boundary segments after the open-boundary copies
+    if (kBC)
+      extbc_row(p.bc, ru, c, R, C, col_int, edge_we, kOpen, simtime, dt, nh, nqx, nqy, ah, aqx, aqy,
+                sc);
 
     // E: wet/dry, CFL dt and store of row r-2 (slot A)
     const int e = r - 2;
@@ -220,8 +228,11 @@ int launch_prime_f64(const StepParams<double>& p, hipStream_t s) {
 
 int launch_step_f64(const StepParams<double>& p, hipStream_t s) {
   dim3 grid(p.nblocks);
-  if (p.open_bc) hipLaunchKernelGGL(k_step_f64<true>, grid, dim3(kLanes), 0, s, p);
-  else hipLaunchKernelGGL(k_step_f64<false>, grid, dim3(kLanes), 0, s, p);
+  const bool bc = p.bc.n > 0;
+  if (p.open_bc && bc) hipLaunchKernelGGL((k_step_f64<true, true>), grid, dim3(kLanes), 0, s, p);
+  else if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false>), grid, dim3(kLanes), 0, s, p);
+  else if (bc) hipLaunchKernelGGL((k_step_f64<false, true>), grid, dim3(kLanes), 0, s, p);
+  else hipLaunchKernelGGL((k_step_f64<false, false>), grid, dim3(kLanes), 0, s, p);
   hipLaunchKernelGGL(k_close_f64, dim3(1), dim3(kLanes), 0, s, p, 1);
   return launch_check("step_f64");
 }
