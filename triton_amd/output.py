@@ -13,9 +13,10 @@ What the reference's loop writes (src/triton_simulate.h:31-134):
   the state at the start, after every step that completes a
   print_observation interval, and at the end.
 
-`run_with_outputs` drives a Simulation (or one member of a Batch) through
-those events: the device stops the run after an observation step
-(set_obs_halt) and the host gathers only the observation cells.
+`run_with_outputs` drives a Simulation through those events (the Ensify
+front-end does the same for the members of a Batch): the device stops the
+run after an observation step (set_obs_halt) and the host gathers only the
+observation cells.
 """
 from __future__ import annotations
 
@@ -123,7 +124,7 @@ def run_with_outputs(sim, case: Case, output_dir=None, cid_dir=None, written=Non
     writes (module docstring); returns the final status.  Without an
     output_dir nothing is written and the run is a plain run().  written: a
     list that receives the paths of the files written at prints."""
-    drv = sim if hasattr(sim, "sample") and hasattr(sim, "status") else _SimDriver(sim)
+    drv = _SimDriver(sim)
     s = case.scalars
     series = None
     if output_dir is not None and int(s.get("time_series_flag", 0)) and case.obs_pos.shape[0] > 0:
