@@ -101,22 +101,37 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   const float hp0 = up ? L.h : R.h;
   const float adz = fabsf(dz);
   const float dzp = hp0 < adz ? (up ? hp0 : -hp0) : dz;
-  float b0 = (0.5f * kGf) * (hp0 - 0.5f * fabsf(dzp)) * dzp * ic;
+  const float b0 = (0.5f * kGf) * (hp0 - 0.5f * fabsf(dzp)) * dzp * ic;
   const float hp = L.h + a0;
   const bool sub = l0 * l2 < 0.f && hp > 0.f && L.h > 0.f && R.h > 0.f;
+  float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
   if (__any(sub)) {
-    // wet-wet clip; without an entropy fix the reference's factors
-    // l/(l+e) are exactly 1
-    float lo = a0 * l0 - L.h * hdxdt;
-    float hi = R.h * hdxdt - a2 * l2;
+    // Wet-wet clip of the source, b0 <- min(max(b0, lo), hi) with
+    // lo = l0 a0 - L.h dx/2dt and hi = R.h dx/2dt - l2 a2 (the reference's
+    // factors l/(l+e) are exactly 1 without an entropy fix), applied to the
+    // strengths: W0 = max(min(W0, Ld), S - Rd), W2 = min(max(W2, S - Ld), Rd)
+    // with S = W0 + W2 and the drains Ld, Rd.  A clip that holds drains the
+    // cell behind (in front) by exactly its depth over two faces, and this
+    // form gives that strength as the product itself; clipping b0 first
+    // would leave it as a difference of terms 1e3 times larger, and FP32
+    // round-off in a cell the FP64 form empties.
+    const float Ld = L.h * hdxdt, Rd = R.h * hdxdt;
+    const float S = W0 + W2;
+    float w0 = fmaxf(fminf(W0, Ld), S - Rd);
+    float w2 = fminf(fmaxf(W2, S - Ld), Rd);
     if (__any(fix0 || fix2)) {
+      float lo = a0 * l0 - Ld;
+      float hi = Rd - a2 * l2;
       lo = fix2 ? (lo + e2 * R.h) * (l0 * rcpf(l0 + e2)) : lo;
       hi = fix0 ? (hi + e0 * L.h) * (l2 * rcpf(l2 + e0)) : hi;
+      const float bc = fminf(fmaxf(b0, lo), hi);
+      const bool f = fix0 || fix2;
+      w0 = f ? l0 * a0 - bc : w0;
+      w2 = f ? l2 * a2 + bc : w2;
     }
-    const float bc = fminf(fmaxf(b0, lo), hi);
-    b0 = sub ? bc : b0;
+    W0 = sub ? w0 : W0;
+    W2 = sub ? w2 : W2;
   }
-  float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
   const float n0 = ub - c, n2 = ub + c;
   // Faces with a dry side (only they can send all mass one way) take the
   // dry-side tests and the flushes; waves without one skip them.
@@ -130,13 +145,14 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     // round-off is ~1e-7 of the terms and would wet the dry cell by ~1e-10 m
     // per step, enough to defeat the wet/dry test; a strength within
     // kFlushRel of the terms it is formed from is zeroed.
-    const bool z0 = fabsf(W0) <= kFlushRel * (fabsf(l0 * a0) + fabsf(b0));
-    const bool z2 = fabsf(W2) <= kFlushRel * (fabsf(l2 * a2) + fabsf(b0));
+    const float bs = sub ? l0 * a0 - W0 : b0;  // the source after the clip
+    const bool z0 = fabsf(W0) <= kFlushRel * (fabsf(l0 * a0) + fabsf(bs));
+    const bool z2 = fabsf(W2) <= kFlushRel * (fabsf(l2 * a2) + fabsf(bs));
     W0 = dry && z0 ? 0.f : W0;
     W2 = dry && z2 ? 0.f : W2;
     const float he = hp + eps;
-    to_left = ((he * l2 - b0) * l2 < 0.f) && R.h < eps;
-    to_right = !to_left && ((he * l0 - b0) * l0 < 0.f) && L.h < eps;
+    to_left = ((he * l2 - bs) * l2 < 0.f) && R.h < eps;
+    to_right = !to_left && ((he * l0 - bs) * l0 < 0.f) && L.h < eps;
   }
   // regular split: back side gets the waves with non-positive speed
   const float B0 = l0 > 0.f ? 0.f : W0, B2 = l2 > 0.f ? 0.f : W2;
