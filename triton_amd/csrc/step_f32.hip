@@ -197,13 +197,17 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   rt = ft;
 }
 
-__device__ __forceinline__ float cell_dt32(float h, float qx, float qy, float cn_dx, float hextra) {
+// A cell's wave speed max(|u|,|v|) + sqrt(g h) when h > hextra, else 0:
+// dt = cn dx / speed is decreasing in the speed, so the smallest dt of a set
+// of cells is cn dx over their largest speed, one reciprocal per wave (the
+// same bits as the min of the per-cell quotients).
+__device__ __forceinline__ float cell_speed32(float h, float qx, float qy, float hextra) {
   if (h > hextra) {
     const float r = rsqf(h);
     const float umax = fmaxf(fabsf(qx), fabsf(qy)) * (r * r);
-    return cn_dx * rcpf(umax + kSqrtGf * (h * r));
+    return umax + kSqrtGf * (h * r);
   }
-  return kMaxDtF;
+  return 0.f;
 }
 
 // The member's fields the step reads, copied to registers once.
@@ -276,7 +280,6 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   const float dtdx = float(dt64 / P.dx);
   const float hdxdt = float(0.5 * P.dx / dt64);
   const float hextra = P.hextra_f;
-  const float cn_dx = P.cn_dx_f;
   const float eps = P.eps_f;
   const bool open_bc = P.open_bc != 0;
   const int R = P.rows, Cc = P.cols;
@@ -295,9 +298,9 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   L.dzx = uni(gptr(P.dzx));
   L.dzy = uni(gptr(P.dzy));
   L.n2 = uni(gptr(P.nman2));
-  L.rid = gptr(P.runoff_id);
+  L.rid = uni(gptr(P.runoff_id));
   L.nz = P.num_runoffs;
-  L.zadd = gptr(P.num_runoffs > 0 ? P.zone_add + cur * P.num_runoffs : nullptr);
+  L.zadd = uni(gptr(P.num_runoffs > 0 ? P.zone_add + cur * P.num_runoffs : nullptr));
   L.rows = R;
   L.cols = Cc;
   // rain depth of this step (FP64 product rate*dt, rounded once)
@@ -321,7 +324,7 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
   float bh = 0.f;
   // external boundary values carried to the south ghost row
   SouthCarry<float> sc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, false};
-  float best = kMaxDtF * 2.f;
+  float fast = -1.f;  // largest wave speed of the cells written (-1: none)
   int ks = 0;  // first source not above the row being updated
 
   // One code path for every row: a row's bits must not depend on where a
@@ -362,8 +365,8 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
         const float m2 = mx * mx + my * my;
         const float mm = sqrtf_hw(m2);
         // tt = dt g n^2 |m| / h^(7/3); q = m * 2/(1+sqrt(1+4tt))
-        const float h73 = __builtin_amdgcn_exp2f(2.3333333f * __builtin_amdgcn_logf(ph));
-        const float tt = dt * kGf * pn2 * mm * rcpf(h73);
+        const float ih73 = __builtin_amdgcn_exp2f(-2.3333333f * __builtin_amdgcn_logf(ph));
+        const float tt = dt * kGf * pn2 * mm * ih73;
         const float k = 2.f * rcpf(1.f + sqrtf_hw(1.f + 4.f * tt));
         kf = mm > eps ? k : 1.f;
       }
@@ -443,7 +446,7 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
         at(qxo, o) = qxq;
         at(qyo, o) = qyq;
         if (maxh && cell_int && ah > at(maxh, o)) at(maxh, o) = ah;
-        best = fminf(best, cell_dt32(ah, qxq, qyq, cn_dx, hextra));
+        fast = fmaxf(fast, cell_speed32(ah, qxq, qyq, hextra));
       }
     }
 
@@ -454,20 +457,23 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
     pdzx = s.dzx; pdzy = s.dzy;
     p0h = c0h; p0x = c0x; p0y = c0y; p1h = c1h; p1x = c1x; p1y = c1y;
   }
-  return best;
+  return fast;
 }
 
-// Wave min of a float dt (positive floats order like their bit patterns);
-// lane 0 publishes it as the FP64 value for the member's close.
-__device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float best, int lane) {
-  unsigned int bits = __float_as_uint(best);
+// Wave max of the lanes' largest speeds (floats >= 0 order like their bit
+// patterns as signed integers, and -1 below them all); lane 0 publishes
+// dt = cn dx / max (1e10 when every cell written is dry, nothing when no cell
+// was written) as the FP64 value for the member's close.
+__device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float fast, float cn_dx, int lane) {
+  int bits = __float_as_int(fast);
   for (int off = 32; off > 0; off >>= 1) {
-    const unsigned int o = __shfl_xor(bits, off, 64);
-    bits = o < bits ? o : bits;
+    const int o = __shfl_xor(bits, off, 64);
+    bits = o > bits ? o : bits;
   }
   if (lane == 0) {
-    const float m = __uint_as_float(bits);
-    publish_min(cc, m >= kMaxDtF * 2.f ? ~0ull : dbits(double(m)));
+    const float m = __int_as_float(bits);
+    const float dt = m > 0.f ? cn_dx * rcpf(m) : kMaxDtF;
+    publish_min(cc, m < 0.f ? ~0ull : dbits(double(dt)));
   }
 }
 
@@ -483,7 +489,7 @@ __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const int p0 = a.wave_first[w], p1 = a.wave_first[w + 1];
   int mem = -1;
   StepCtrl* cc = nullptr;
-  float acc = kMaxDtF * 2.f;
+  float acc = -1.f, cn_dx = 0.f;
   for (int pi = p0; pi < p1; pi++) {
     const Piece pc = a.pieces[pi];
     const InstF32& P = a.inst[pc.member];
@@ -491,14 +497,15 @@ __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
     const int cur = __builtin_amdgcn_readfirstlane(*P.cursor);
     if (P.ctrl[cur].halt) continue;
     if (pc.member != mem) {
-      if (cc) publish_min_f32(cc, acc, lane);
+      if (cc) publish_min_f32(cc, acc, cn_dx, lane);
       mem = pc.member;
       cc = P.ctrl + cur;
-      acc = kMaxDtF * 2.f;
+      cn_dx = P.cn_dx_f;
+      acc = -1.f;
     }
-    acc = fminf(acc, step_piece<kBC>(P, cur, pc.bx, pc.row_first, pc.row_last, lane));
+    acc = fmaxf(acc, step_piece<kBC>(P, cur, pc.bx, pc.row_first, pc.row_last, lane));
   }
-  if (cc) publish_min_f32(cc, acc, lane);
+  if (cc) publish_min_f32(cc, acc, cn_dx, lane);
 }
 
 // Initial dt of one member over every cell (ghosts included).
@@ -506,10 +513,10 @@ __global__ __launch_bounds__(64) void k_prime_f32(const InstF32* inst, int membe
   const InstF32& P = inst[member];
   const int cur = *P.cursor;
   const int n = P.rows * P.cols;
-  float best = kMaxDtF * 2.f;
+  float fast = -1.f;
   for (int id = blockIdx.x * kLanes + threadIdx.x; id < n; id += gridDim.x * kLanes)
-    best = fminf(best, cell_dt32(P.h[cur][id], P.qx[cur][id], P.qy[cur][id], P.cn_dx_f, P.hextra_f));
-  publish_min_f32(P.ctrl + cur, best, int(threadIdx.x));
+    fast = fmaxf(fast, cell_speed32(P.h[cur][id], P.qx[cur][id], P.qy[cur][id], P.hextra_f));
+  publish_min_f32(P.ctrl + cur, fast, P.cn_dx_f, int(threadIdx.x));
 }
 
 // One thread per member (members[i], or first + i when members is NULL):
