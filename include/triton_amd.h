@@ -245,12 +245,12 @@ int tr_batch_create(int64_t hbm_budget_bytes, int32_t waves_per_simd, tr_batch *
 int64_t tr_batch_member_bytes(const tr_case_view *v);
 int tr_batch_add(tr_batch *b, const tr_case_view *v, int32_t *id);
 /* tune->dry_eps applies per member; the other tuning fields are the batch's. */
+int tr_batch_add_ex(tr_batch *b, const tr_case_view *v, const tr_tuning *tune, int32_t *id);
 /* With on != 0 the member stops after every step that completes a print
  * interval (print_id advances) until the next tr_batch_arm, so its state can
  * be written out at print times as the reference's loop does
  * (src/triton_simulate.h:92-110). */
 int tr_batch_set_print_halt(tr_batch *b, int32_t id, int32_t on);
-int tr_batch_add_ex(tr_batch *b, const tr_case_view *v, const tr_tuning *tune, int32_t *id);
 /* Default rows per wave of the FP64 fused step for a padded rows x cols grid
  * (enough waves to fill the device; long strips otherwise). */
 int32_t tr_default_strip_rows(int32_t rows, int32_t cols);
