@@ -107,11 +107,10 @@ struct tr_batch {
   CtrlPair* h_ctrl = nullptr;  // pinned mirror
   int cap = 0;
   Piece* d_pieces = nullptr;
-  int* d_first = nullptr;
   int* d_members = nullptr;    // [cap] active members, in partition order
   int nmembers = 0;
   int any_bc = 0;              // some member in the partition has external boundaries
-  int piece_cap = 0, first_cap = 0;
+  int piece_cap = 0;
   int nwork = 0;               // waves per launch
   int npieces = 0;
   int simd_count = 1024;
@@ -138,7 +137,6 @@ struct tr_batch {
     hipFree(d_inst);
     hipFree(d_ctrl);
     hipFree(d_pieces);
-    hipFree(d_first);
     hipFree(d_members);
     hipFree(d_part);
     hipFree(d_sum);
@@ -314,7 +312,6 @@ int rebuild_work(tr_batch* b) {
     if (b->m[size_t(i)].inst.bc.n > 0) b->any_bc = 1;
   }
   std::vector<Piece> pieces;
-  std::vector<int> first;
   int nw = 0;
   if (total > 0) {
     const int64_t want = int64_t(b->simd_count) * b->waves_per_simd;
@@ -342,29 +339,20 @@ int rebuild_work(tr_batch* b) {
       const int nb = int((P.rows + H - 1) / H);
       for (int k = 0; k < nb; k++) {
         const int r0 = int(int64_t(P.rows) * k / nb), r1 = int(int64_t(P.rows) * (k + 1) / nb);
-        for (int bx = 0; bx < P.nbx; bx++) {
-          first.push_back(int(pieces.size()));
-          pieces.push_back(Piece{i, bx, r0, r1 - 1});
-        }
+        for (int bx = 0; bx < P.nbx; bx++) pieces.push_back(Piece{i, bx, r0, r1 - 1});
       }
     }
-    nw = int(first.size());
+    nw = int(pieces.size());
   }
-  first.push_back(int(pieces.size()));
-  const size_t np = std::max<size_t>(pieces.size(), 1), nf = first.size();
-  if (np > size_t(b->piece_cap) || nf > size_t(b->first_cap)) {
+  const size_t np = std::max<size_t>(pieces.size(), 1);
+  if (np > size_t(b->piece_cap)) {
     hipFree(b->d_pieces);
-    hipFree(b->d_first);
     b->d_pieces = nullptr;
-    b->d_first = nullptr;
     b->piece_cap = int(np * 2);
-    b->first_cap = int(nf * 2);
     TR_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_pieces), sizeof(Piece) * size_t(b->piece_cap)));
-    TR_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_first), sizeof(int) * size_t(b->first_cap)));
   }
   if (!pieces.empty())
     TR_HIP(hipMemcpy(b->d_pieces, pieces.data(), sizeof(Piece) * pieces.size(), hipMemcpyHostToDevice));
-  TR_HIP(hipMemcpy(b->d_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice));
   if (!order.empty())
     TR_HIP(hipMemcpy(b->d_members, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
   b->nmembers = int(order.size());
@@ -700,7 +688,7 @@ int tr_batch_arm(tr_batch* b, double t_stop, int64_t max_steps) {
 namespace {
 
 bool same_args(const BatchArgs& x, const BatchArgs& y) {
-  return x.inst == y.inst && x.pieces == y.pieces && x.wave_first == y.wave_first &&
+  return x.inst == y.inst && x.pieces == y.pieces &&
          x.nwaves == y.nwaves && x.members == y.members && x.nmembers == y.nmembers &&
          x.any_bc == y.any_bc;
 }
@@ -745,7 +733,7 @@ int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
     return TR_ERR_INVALID;
   }
   if (b->table_dirty) RC(rebuild_work(b));
-  BatchArgs a{b->d_inst, b->d_pieces, b->d_first, b->nwork, b->d_members, b->nmembers, b->any_bc};
+  BatchArgs a{b->d_inst, b->d_pieces, b->nwork, b->d_members, b->nmembers, b->any_bc};
   int64_t k = 0;
   if (b->graph_steps > 0 && a.nwaves > 0 && nsteps >= b->graph_steps) {
     if (!b->gexec || b->gsteps_built != b->graph_steps || !same_args(a, b->gargs)) RC(build_graph(b, a));

@@ -55,8 +55,8 @@ struct InstF32 {
   ExtBC bc;                 // external boundary segments (bc.n == 0: none)
 };
 
-// A piece is a run of rows of one column strip of one member; waves take
-// equal numbers of rows, so a wave walks one or a few consecutive pieces.
+// A piece is a run of rows of one column strip of one member, and the work
+// of one wave.
 struct Piece {
   int member;
   int bx;          // column strip
@@ -66,8 +66,7 @@ struct Piece {
 
 struct BatchArgs {
   const InstF32* inst;
-  const Piece* pieces;
-  const int* wave_first;     // pieces of wave w: [wave_first[w], wave_first[w+1])
+  const Piece* pieces;       // [nwaves]: wave w steps pieces[w]
   int nwaves;
   const int* members;        // members in the partition (the close kernel's list)
   int nmembers;

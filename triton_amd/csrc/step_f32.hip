@@ -272,10 +272,8 @@ __device__ __forceinline__ Raw load_row(const Loc& L, int r, unsigned col4) {
 // One member's piece: rows [row_first, row_last] of column strip bx, with
 // the two-row halo recomputed at its ends.  Returns the piece's dt min bits.
 template <bool kBC>
-__device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, int e_min, int e_max,
-                                            int lane) {
-  const StepCtrl& CC = P.ctrl[cur];
-  const double dt64 = CC.dt;
+__device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt64, double simtime,
+                                            int bx, int e_min, int e_max, int lane) {
   const float dt = float(dt64);
   const float dtdx = float(dt64 / P.dx);
   const float hdxdt = float(0.5 * P.dx / dt64);
@@ -425,8 +423,8 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, int bx, i
     }
     // external boundary segments after the open-boundary copies
     if (kBC && P.bc.n > 0)
-      extbc_row(P.bc, ru, c, R, Cc, col_int, edge_we, open_bc, CC.simtime, dt64, nh, nqx, nqy, ah,
-                aqx, aqy, sc);
+      extbc_row(P.bc, ru, c, R, Cc, col_int, edge_we, open_bc, simtime, dt64, nh, nqx, nqy, ah, aqx,
+                aqy, sc);
 
     // wet/dry, dt and store of row r-2
     const int e = r - 2;
@@ -477,8 +475,10 @@ __device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float fast, float 
   }
 }
 
-// Batched step: wave w (placed XCD-contiguously) walks its pieces in order
-// and publishes one dt min per run of pieces of the same member.
+// Batched step: wave w (placed XCD-contiguously) steps piece w and publishes
+// its dt min.  The cursor and both control slots are read together, so the
+// rows' loads wait on three dependent reads (piece, member descriptor,
+// cursor) -- what a small grid's step time is made of.
 // 5 waves per SIMD: the step fits in 96 VGPRs without spills.
 // kBC: some member has external boundary segments (a variant of its own, so
 // the common one keeps its registers).
@@ -486,26 +486,18 @@ template <bool kBC>
 __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const int w = xcd_slot(int(blockIdx.x), a.nwaves);
   const int lane = threadIdx.x;
-  const int p0 = a.wave_first[w], p1 = a.wave_first[w + 1];
-  int mem = -1;
-  StepCtrl* cc = nullptr;
-  float acc = -1.f, cn_dx = 0.f;
-  for (int pi = p0; pi < p1; pi++) {
-    const Piece pc = a.pieces[pi];
-    const InstF32& P = a.inst[pc.member];
-    // wave-uniform: keeps the member's array bases in scalar registers
-    const int cur = __builtin_amdgcn_readfirstlane(*P.cursor);
-    if (P.ctrl[cur].halt) continue;
-    if (pc.member != mem) {
-      if (cc) publish_min_f32(cc, acc, cn_dx, lane);
-      mem = pc.member;
-      cc = P.ctrl + cur;
-      cn_dx = P.cn_dx_f;
-      acc = -1.f;
-    }
-    acc = fmaxf(acc, step_piece<kBC>(P, cur, pc.bx, pc.row_first, pc.row_last, lane));
-  }
-  if (cc) publish_min_f32(cc, acc, cn_dx, lane);
+  const Piece pc = a.pieces[w];
+  const InstF32& P = a.inst[pc.member];
+  // wave-uniform: keeps the member's array bases in scalar registers
+  const int cur = __builtin_amdgcn_readfirstlane(*P.cursor);
+  const StepCtrl& c0 = P.ctrl[0];
+  const StepCtrl& c1 = P.ctrl[1];
+  const int halt0 = c0.halt, halt1 = c1.halt;
+  const double dt0 = c0.dt, dt1 = c1.dt, t0 = c0.simtime, t1 = c1.simtime;
+  if (cur ? halt1 : halt0) return;
+  const float fast = step_piece<kBC>(P, cur, cur ? dt1 : dt0, cur ? t1 : t0, pc.bx, pc.row_first,
+                                     pc.row_last, lane);
+  publish_min_f32(P.ctrl + cur, fast, P.cn_dx_f, lane);
 }
 
 // Initial dt of one member over every cell (ghosts included).
