@@ -31,8 +31,15 @@
  *   orc_run observation rows and the time_step= dt sum: no reference
  *       fixture; tests/test_oracle_invariants.py derives the recorded steps
  *       and the sum again in Python from a run that records every step.
- *   orc_compute_extbc_values types 0/1/2: parity unpinned (no fixture, no
- *       closed form beyond the defining formula).
+ *   orc_compute_extbc_values types 0/1/2: no reference fixture; pinned by
+ *       closed forms -- type 0 on every edge cell adds nothing to the open
+ *       boundaries' copies and leaves every ghost equal to its cell under
+ *       walls; type 1 with a linear level series gives the level at the
+ *       step's midpoint minus the bed (clamped), first / last level outside
+ *       the series; type 2 gives Manning's normal-flow discharge
+ *       h^(5/3) sqrt(S)/n (n^2 floored at 1e-6) along each edge's outward
+ *       normal.  Nine planted slips in these (time, exponent, floor, sign,
+ *       ghost index, corner, factors, bed, clamp) each fail a test.
  */
 #ifndef TRITON_ORACLE_H
 #define TRITON_ORACLE_H

@@ -22,6 +22,11 @@ index, a wrong neighbour) would break.
                      test picks out of a run that records every step; the
                      time_step= dt sum matches the per-step dts.
   hotstart .cfg      the reference's key replacement, including its quirks.
+  external bounds    type 0 adds nothing to open-boundary copies and copies
+                     each cell to its ghosts under walls; type 1 holds a
+                     linear level series' value at the step's midpoint;
+                     type 2 gives Manning's normal-flow discharge along the
+                     edge's outward normal.
 """
 import math
 
@@ -210,3 +215,108 @@ def test_cfg_for_print_replacement():
     plain = "sim_start_time=0\ncheckpoint_id=0\ntime_step=0.01\nit_count=0\n"
     assert cfg_for_print(plain, 1.0 / 3.0, 1, 0.0, 7) == (
         "sim_start_time=0.33333333333333331\ncheckpoint_id=1\ntime_step=0.00000000000000000\nit_count=7\n")
+
+
+# ---- external boundary types 0, 1 and 2 (src/kernels.h:257-420) ----------
+# No reference fixture exercises them (allatoona uses type 3 only); each has
+# a closed form that a slip in the restatement (a wrong ghost index, factor
+# sign, time, exponent or n^2 floor) breaks.
+
+def edge_cells(nr, nc):
+    """Every cell of the unpadded grid's outer ring, corners included."""
+    return sorted({(r, c) for r in range(nr) for c in range(nc) if r in (0, nr - 1) or c in (0, nc - 1)})
+
+
+def test_extbc_type0_adds_nothing_to_open_boundaries():
+    """Type 0 copies a cell to its ghost (and, at a corner, to the ghost
+    beside it in the row), which is what the open-boundary copies just did:
+    a run with type 0 on every edge cell equals the run without segments."""
+    from test_gpu_sim import with_extbc
+    nr, nc = 24, 31
+    z = np.random.default_rng(3).uniform(0.0, 0.3, (nr, nc))
+    h = np.maximum(np.where(np.add.outer(np.arange(nr), np.arange(nc)) < 20, 2.0, 0.7) - z, 0.0)
+    plain = Case.from_arrays(dem=z, h=h, nman=0.02, dx=1.5, sim_duration=3.0, open_boundaries=1)
+    seg = with_extbc(plain.copy(), [(rc, 0, None) for rc in edge_cells(nr, nc)])
+    a, ra = run(plain)
+    b, rb = run(seg)
+    assert ra["steps"] == rb["steps"] >= 20
+    for k in ("h", "qx", "qy"):
+        np.testing.assert_array_equal(b.state[k], a.state[k])
+    # With walls nothing else writes the ghosts: after a step every ghost of
+    # a type-0 cell, the corner's second ghost included, holds the cell.
+    walls = with_extbc(Case.from_arrays(dem=z, h=h, nman=0.02, dx=1.5, sim_duration=3.0),
+                       [(rc, 0, None) for rc in edge_cells(nr, nc)])
+    w, _ = run(walls, max_steps=1)
+    for r, c in edge_cells(nr, nc):
+        R, Cc = r + 1, c + 1
+        ghosts = [(R + dr, Cc + dc) for dr, dc, edge in ((-1, 0, r == 0), (1, 0, r == nr - 1),
+                                                          (0, -1, c == 0), (0, 1, c == nc - 1)) if edge]
+        for g in ghosts:
+            assert all(w.state[k][g] == w.state[k][R, Cc] for k in ("h", "qx", "qy")), (r, c, g)
+
+
+def test_extbc_type1_level_at_the_step_midpoint():
+    """A level series on a line L(t) = 1.5 + 0.01 t: after one step from
+    t0 the edge cell holds L(t0 + dt/2) - z (clamped at 0) and so does its
+    ghost; a series that starts after the step holds its first level, one
+    that ended before it its last."""
+    from test_gpu_sim import with_extbc
+    nr, nc = 12, 15
+    z = np.linspace(0.2, 1.8, nr * nc).reshape(nr, nc)
+    case = Case.from_arrays(dem=z, h=np.maximum(1.0 - z, 0.0) + 0.05, dx=2.0, sim_duration=50.0)
+    case.scalars["sim_start"] = 7.0
+    line = [(t, 1.5 + 0.01 * t) for t in (0.0, 10.0, 20.0, 40.0)]
+    future = [(100.0, 0.9), (200.0, 3.0)]
+    past = [(0.0, 3.0), (1.0, 1.1)]
+    ent = [((0, c), 1, line) for c in range(1, nc - 1)]            # north edge
+    ent += [((r, nc - 1), 1, line) for r in range(1, nr - 1)]      # east edge, z up to 1.8
+    ent += [((nr - 1, 3), 1, future), ((nr - 1, 6), 1, past)]      # south edge
+    ent += [((0, 0), 1, line), ((nr - 1, nc - 1), 1, line)]        # two corners
+    case = with_extbc(case, ent)
+    o, r = run(case, max_steps=1)
+    t0, dt = 7.0, r["dt"]
+    assert r["steps"] == 1 and r["simtime"] == pytest.approx(t0 + dt)
+    h, zp = o.state["h"], case.arrays["dem"]
+    lvl = 1.5 + 0.01 * (t0 + 0.5 * dt)
+    for (rr, cc), _, series in ent:
+        R, Cc = rr + 1, cc + 1
+        want = max((lvl if series is line else series[0 if series is future else -1][1]) - zp[R, Cc], 0.0)
+        assert h[R, Cc] == pytest.approx(want, abs=1e-12), (rr, cc)
+        ghosts = [(R + dr, Cc + dc) for dr, dc, edge in ((-1, 0, rr == 0), (1, 0, rr == nr - 1),
+                                                          (0, -1, cc == 0), (0, 1, cc == nc - 1)) if edge]
+        assert ghosts and all(h[g] == h[R, Cc] for g in ghosts), (rr, cc)
+    assert any(h[r + 1, nc] == 0.0 for r in range(1, nr - 1))  # east cells above the level dry
+
+
+@pytest.mark.parametrize("n", [0.03, 0.0])
+def test_extbc_type2_manning_normal_flow(n):
+    """Type 2 sets the edge discharge to Manning's normal flow for the
+    cell's depth, q = h * h^(2/3) sqrt(S) / n (n^2 floored at 1e-6), along
+    the edge's outward normal: +x east, -x west, +y north, -y south, with
+    the tangential component zeroed; the ghost gets the same."""
+    from test_gpu_sim import with_extbc
+    nr, nc = 14, 11
+    rng = np.random.default_rng(11)
+    case = Case.from_arrays(dem=np.zeros((nr, nc)), h=rng.uniform(0.4, 1.2, (nr, nc)), nman=n,
+                            dx=1.0, sim_duration=10.0)
+    S = 0.004
+    ent = [((0, c), 2, S) for c in range(2, nc - 2)] + [((nr - 1, c), 2, S) for c in range(2, nc - 2)]
+    ent += [((r, 0), 2, S) for r in range(2, nr - 2)] + [((r, nc - 1), 2, S) for r in range(2, nr - 2)]
+    case = with_extbc(case, ent)
+    o, r = run(case, max_steps=1)
+    h, qx, qy = o.state["h"], o.state["qx"], o.state["qy"]
+    nn = max(n * n, 1e-6) ** 0.5
+    for (rr, cc), _, _ in ent:
+        R, Cc = rr + 1, cc + 1
+        q = h[R, Cc] * h[R, Cc] ** (2.0 / 3.0) * math.sqrt(S) / nn
+        if rr == 0:
+            want, ghost = (0.0, q), (R - 1, Cc)
+        elif rr == nr - 1:
+            want, ghost = (0.0, -q), (R + 1, Cc)
+        elif cc == 0:
+            want, ghost = (-q, 0.0), (R, Cc - 1)
+        else:
+            want, ghost = (q, 0.0), (R, Cc + 1)
+        got = (qx[R, Cc], qy[R, Cc])
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-15), (rr, cc)
+        assert (h[ghost], qx[ghost], qy[ghost]) == (h[R, Cc], got[0], got[1])
