@@ -369,7 +369,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): every run of this build "
-              "uses a single MI355X; how the path shards over GPUs is described in DESIGN.md",
+              "uses a single MI355X (ensembles and served requests would shard by member, one batch "
+              "per GPU; one grid by 2-D blocks with a two-cell halo exchange, not built)",
               file=sys.stderr)
         sys.exit(2)
     import torch
