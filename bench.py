@@ -203,28 +203,8 @@ def run_ensemble(args):
         res["fp64_plain"] = fp64_plain(members, args.spinup_s, tuned + args.warmup, fp64_steps,
                                        keep=spun if args.cpu_seconds > 0 else None)
         res["speedup_fp32_batched_vs_fp64_plain"] = round(gcups / res["fp64_plain"]["value"], 3)
-    res["cases"] = spun or [spun_up(m, args.spinup_s) for m in members[:8]] if args.cpu_seconds > 0 else members
+    res["cases"] = spun or [scenarios.spun_up(m, args.spinup_s) for m in members[:8]] if args.cpu_seconds > 0 else members
     return res
-
-
-def restart_at(case, state, simtime):
-    """`case` with its state replaced by an FP64 state at `simtime` (the
-    reference's hotstart: sim_start and the initial rasters)."""
-    c = case.copy()
-    for k in ("h", "qx", "qy"):
-        c.arrays[k] = state[k][1:-1, 1:-1].copy() if state[k].shape != c.arrays[k].shape else state[k].copy()
-    c.scalars["sim_start"] = float(simtime)
-    return c
-
-
-def spun_up(case, t_stop):
-    """`case` advanced to t_stop by the FP64 fused form."""
-    import triton_amd as ta
-    if t_stop <= 0:
-        return case
-    with ta.Simulation(case, ta.FORM_F64_FUSED) as sim:
-        st = sim.run(t_stop=t_stop)
-        return restart_at(case, sim.state(), st["simtime"])
 
 
 def fp64_plain(members, spinup_s, warmup, steps, keep=None):
@@ -234,6 +214,7 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
     each member's stream.  keep: a list that receives the first members'
     spun-up states as cases (the CPU baseline's sample)."""
     import torch
+    from triton_amd import scenarios
     import triton_amd as ta
     total_ms = 0.0
     total_cells = 0
@@ -243,7 +224,7 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
         if spinup_s > 0:
             st = sim.run(t_stop=spinup_s)
             if keep is not None and len(keep) < 8:
-                keep.append(restart_at(m, sim.state(), st["simtime"]))
+                keep.append(scenarios.restart_at(m, sim.state(), st["simtime"]))
         base = int(sim.sync()["steps"])
         sim.run(max_steps=warmup)
         sim.arm(max_steps=steps)
@@ -331,7 +312,7 @@ def run_single(args):
     if args.workload == "watershed" and args.spinup_s > 0:
         st = sim.run(t_stop=args.spinup_s)
         desc["spinup_s"] = args.spinup_s
-        case = restart_at(case, sim.state(), st["simtime"])  # the CPU baseline's sample
+        case = scenarios.restart_at(case, sim.state(), st["simtime"])  # the CPU baseline's sample
     sim.run(max_steps=args.warmup)
     st0 = sim.sync()
     sim.arm(max_steps=args.steps)

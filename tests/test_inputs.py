@@ -123,6 +123,23 @@ def test_runoff_units_and_zones(tmp_path):
     assert c.scalars["init_dt"] == pytest.approx(0.01 * 600.0)
 
 
+@pytest.mark.parametrize("t0,row", [(0.0, 0), (450.0, 0), (900.0, 0), (901.0, 1), (3600.0, 1), (4000.0, 0)])
+def test_runoff_start_row(tmp_path, t0, row):
+    """A run starting at t0 applies first the row j with t[j] < t0 <= t[j+1]
+    (else row 0) and then advances one row per step (src/triton_init.h:
+    1303-1310, src/triton_simulate.h:246-258).  The .cfg loader and a case
+    whose start time is changed afterwards (a restart from arrays) agree."""
+    c = load(write_case(tmp_path, cfg_extra=f"sim_start_time={t0}"))
+    assert c.scalars["roff_row0"] == row
+    assert c.view().roff_row0 == row
+    d = tmp_path / "from_zero"
+    d.mkdir()
+    c0 = load(write_case(d))
+    c0.scalars["sim_start"] = t0
+    assert c0.view().roff_row0 == row
+    assert ta.runoff_row0(c0.arrays["roff_time"], t0) == row
+
+
 def test_missing_input_is_an_input_error(tmp_path):
     p = write_case(tmp_path)
     (tmp_path / "flow.hyg").unlink()

@@ -179,6 +179,15 @@ def _ptr(a: np.ndarray | None, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+def runoff_row0(roff_time, t0: float) -> int:
+    """Runoff row a run starting at t0 applies first: the last j with
+    t[j] < t0 <= t[j+1], else 0 (src/triton_init.h:1303-1310); later rows
+    follow one per step (src/triton_simulate.h:246-258)."""
+    t = np.asarray(roff_time, np.float64)
+    j = np.nonzero((t[:-1] < t0) & (t0 <= t[1:]))[0]
+    return int(j[-1]) if j.size else 0
+
+
 class Case:
     """A loaded or synthesized case: padded FP64 arrays plus forcing tables.
 
@@ -359,6 +368,7 @@ class Case:
         if "roff_time" in a:
             v.num_runoffs = a["roff_rate"].shape[0]
             v.roff_nrows = a["roff_time"].shape[0]
+            v.roff_row0 = runoff_row0(a["roff_time"], v.sim_start)
             v.roff_time = _ptr(a["roff_time"], C.c_double)
             v.roff_rate = _ptr(a["roff_rate"], C.c_double)
             v.runoff_id = _ptr(a.get("runoff_id"), C.c_int32)

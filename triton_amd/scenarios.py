@@ -181,3 +181,23 @@ def ensemble(count: int = 64, hours: float = 2.0, seed: int = 4, sizes=None):
         out.append(watershed(n=side, seed=4000 + i, hours=hours, rain_peak_mmh=peak, inflow_peaks=(),
                              manning_scale=ms))
     return out
+
+
+def restart_at(case, state, simtime):
+    """`case` with its state replaced by an FP64 state at `simtime` (the
+    reference's hotstart: sim_start and the initial rasters)."""
+    c = case.copy()
+    for k in ("h", "qx", "qy"):
+        c.arrays[k] = state[k][1:-1, 1:-1].copy() if state[k].shape != c.arrays[k].shape else state[k].copy()
+    c.scalars["sim_start"] = float(simtime)
+    return c
+
+
+def spun_up(case, t_stop):
+    """`case` advanced to t_stop by the FP64 fused form."""
+    import triton_amd as ta
+    if t_stop <= 0:
+        return case
+    with ta.Simulation(case, ta.FORM_F64_FUSED) as sim:
+        st = sim.run(t_stop=t_stop)
+        return restart_at(case, sim.state(), st["simtime"])
