@@ -28,8 +28,7 @@ import os
 
 import numpy as np
 import pytest
-from scipy.ndimage import binary_dilation
-
+from accuracy import errors, mean_dh_vs_analytic
 from oracle.pyoracle import Oracle
 import triton_amd as ta
 from triton_amd import scenarios
@@ -69,27 +68,6 @@ def fp32_input_floor(name):
     return float(np.abs(o.state["h"][1:-1, 1:-1] - ref["h"][1:-1, 1:-1]).max())
 
 
-def errors(case, ref, got, hextra):
-    out = {}
-    for k in ("h", "qx", "qy"):
-        d = np.abs(got[k][1:-1, 1:-1] - ref[k][1:-1, 1:-1])
-        out[f"max_d{k}"] = float(d.max())
-        out[f"mean_d{k}"] = float(d.mean())
-    m_ref = float(ref["h"][1:-1, 1:-1].sum())
-    m_got = float(got["h"][1:-1, 1:-1].sum())
-    out["rel_mass"] = abs(m_got - m_ref) / m_ref if m_ref > 0 else 0.0
-    wr, wg = ref["h"][1:-1, 1:-1] > hextra, got["h"][1:-1, 1:-1] > hextra
-    out["wet_mismatch"] = int((wr != wg).sum())
-    out["cells"] = int(wr.size)
-    edges = (wr ^ binary_dilation(wr)) | (wg ^ binary_dilation(wg)) | (wr != wg)
-    band = binary_dilation(edges, iterations=3)
-    inner = wr & wg & ~band
-    dh = np.abs(got["h"][1:-1, 1:-1] - ref["h"][1:-1, 1:-1])
-    out["max_dh_inner"] = float(dh[inner].max()) if inner.any() else 0.0
-    out["max_dh_shoreline"] = float(dh[band].max()) if band.any() else 0.0
-    return out
-
-
 @pytest.mark.parametrize("form", list(FORMS))
 @pytest.mark.parametrize("name", list(CASES))
 def test_error_against_oracle(gpu, name, form):
@@ -105,9 +83,7 @@ def test_error_against_oracle(gpu, name, form):
     if name.startswith("thacker"):
         n = case.rows - 2
         _, he, _, _, _ = scenarios.thacker_exact(n=n, t=st["simtime"])
-        hg = got["h"][1:-1, 1:-1]
-        wet = (he > 0) | (hg > case.scalars["hextra"])
-        e["mean_dh_vs_analytic"] = float(np.abs(hg - he)[wet].mean())
+        e["mean_dh_vs_analytic"] = mean_dh_vs_analytic(got["h"][1:-1, 1:-1], he, case.scalars["hextra"])
     path = os.environ.get("TR_ACCURACY_REPORT")
     if path:
         with open(path, "a") as f:

@@ -20,7 +20,9 @@
 //     reference's FP64 value; its FP32 build uses 1e-6, src/constants.h:43);
 //   * bed tests z1+h1<z2 become h1<dz on the stored face difference;
 //   * the semi-implicit friction uses q = 2m/(1+sqrt(1+4tt)), the same value
-//     as -0.5(m - m sqrt(1+4tt))/tt without its cancellation for small tt.
+//     as -0.5(m - m sqrt(1+4tt))/tt without its cancellation for small tt;
+//   * the updated depth (flux update and rain) is carried as two floats and
+//     rounded to FP32 stochastically, which keeps mass (sround).
 #include <hip/hip_runtime.h>
 
 #include "batch.hpp"
@@ -47,6 +49,45 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 // h*rsq(h) must stay finite.
 __device__ __forceinline__ float rsqf(float x) { return __builtin_amdgcn_rsqf(fmaxf(x, 1e-30f)); }
 __device__ __forceinline__ float sqrtf_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// a + b == s + lo exactly (Knuth's TwoSum), s returned.
+__device__ __forceinline__ float two_sum(float a, float b, float& lo) {
+#pragma clang fp contract(off)
+  const float s = a + b;
+  const float bb = s - a;
+  lo = (a - (s - bb)) + (b - bb);
+  return s;
+}
+
+// The reference's h < 0 -> 0 on the two-float depth hs + he.
+__device__ __forceinline__ void clamp_neg(float& hs, float& he) {
+  const bool neg = hs < 0.f || (hs == 0.f && he < 0.f);
+  hs = neg ? 0.f : hs;
+  he = neg ? 0.f : he;
+}
+
+// Stored depths are rounded stochastically.  With round-to-nearest the FP32
+// form lost mass steadily on deep water, where a step changes a cell by a few
+// ulps (an ulp is 9.5e-7 m at 10 m): -1.3e-5 relative mass over two periods
+// of the 1024^2 Thacker bowl, +8.6e-6 at 512^2, linear in the step count, the
+// same with every dry threshold and with IEEE division, square root and no
+// FMA contraction; one FP32 step from an FP64 state showed no such loss, one
+// from an FP32-evolved state did.  Rounded stochastically: -2.9e-7 and
+// +2.3e-7.  hs + he (|he| <= ulp(hs)) goes to the float one ulp away in he's
+// direction with probability |he| / ulp, else stays hs, so the expected
+// stored depth is hs + he; at hs = 2^k the step down is two spacings at half
+// the probability, the same expected value.  u in [0, 1) is a per-cell Weyl
+// sequence over the iterations (golden-ratio increment), so each cell's
+// decisions are equidistributed in time; it depends only on the member's
+// cell and iteration count, so a member's bits do not depend on the batch or
+// the partition, and a hotstart continues the same sequence.
+__device__ __forceinline__ float sround(float hs, float he, float u) {
+  const float ulp = __int_as_float(__float_as_int(hs) & 0x7f800000) * 0x1p-23f;
+  return u * ulp < fabsf(he) ? hs + copysignf(ulp, he) : hs;
+}
+constexpr unsigned kDitherCol = 0x85EBCA77u;
+constexpr unsigned kDitherRow = 0xC2B2AE3Du;
+constexpr unsigned kDitherStep = 0x9E3779B9u;  // 2^32 / golden ratio
 
 struct Side {
   float h, qn, qt, un, ut, sq;
@@ -273,7 +314,7 @@ __device__ __forceinline__ Raw load_row(const Loc& L, int r, unsigned col4) {
 // the two-row halo recomputed at its ends.  Returns the piece's dt min bits.
 template <bool kBC>
 __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt64, double simtime,
-                                            int bx, int e_min, int e_max, int lane) {
+                                            unsigned dither_step, int bx, int e_min, int e_max, int lane) {
   const float dt = float(dt64);
   const float dtdx = float(dt64 / P.dx);
   const float hdxdt = float(0.5 * P.dx / dt64);
@@ -305,6 +346,7 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
   L.add0 = P.num_runoffs > 0 ? float(L.zadd[0]) : 0.f;
 
   const int c = kOutCols * bx - 1 + lane;
+  const unsigned dither_col = unsigned(c) * kDitherCol;
   const unsigned col4 = unsigned(min(max(c, 0), Cc - 1)) * 4u;
   const bool col_int = c >= 1 && c <= Cc - 2;
   const bool out_lane = lane >= 2 && lane <= kOutCols + 1 && col_int;
@@ -354,8 +396,18 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
     float nh = ph, nqx = pqx, nqy = pqy;
     const bool ru_int = ru >= 1 && ru <= R - 2;
     if (ru_int && col_int) {
-      float hnew = ph - dtdx * (p0h + (p1h + y1h));
-      hnew = hnew < 0.f ? 0.f : hnew;
+      // depth ph - dtdx*S carried as hs + he, he the rounding error
+      float hs, he;
+      {
+#pragma clang fp contract(off)
+        const float S = p0h + (p1h + y1h);
+        const float d = dtdx * S;
+        const float de = fmaf(dtdx, S, -d);  // dtdx*S == d + de exactly
+        float e;
+        hs = two_sum(ph, -d, e);
+        he = e - de;
+      }
+      clamp_neg(hs, he);
       const float mx = pqx - dtdx * (p0x + (p1x + y1x));
       const float my = pqy - dtdx * (p0y + (p1y + y1y));
       float kf = 1.f;
@@ -368,17 +420,19 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
         const float k = 2.f * rcpf(1.f + sqrtf_hw(1.f + 4.f * tt));
         kf = mm > eps ? k : 1.f;
       }
-      const bool wet = hnew >= hextra;
+      const bool wet = hs >= hextra;
       nqx = wet ? mx * kf : 0.f;
       nqy = wet ? my * kf : 0.f;
       if (L.nz > 0) {
-        hnew += padd;
-        const bool low = hnew < hextra;
-        hnew = hnew < 0.f ? 0.f : hnew;
+        float e;
+        hs = two_sum(hs, padd, e);
+        he += e;
+        const bool low = hs < hextra;
+        clamp_neg(hs, he);
         nqx = low ? 0.f : nqx;
         nqy = low ? 0.f : nqy;
       }
-      nh = hnew;
+      nh = sround(hs, he, float(dither_col + (dither_step + unsigned(ru) * kDitherRow)) * 0x1p-32f);
     }
     // x face (c, c+1) of row r: back side stays in this lane (rhs*0), the
     // front side goes to lane c+1 (rhs*1).  Faces outside the reference's
@@ -494,8 +548,10 @@ __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const StepCtrl& c1 = P.ctrl[1];
   const int halt0 = c0.halt, halt1 = c1.halt;
   const double dt0 = c0.dt, dt1 = c1.dt, t0 = c0.simtime, t1 = c1.simtime;
+  const long long n0 = c0.step, n1 = c1.step;
   if (cur ? halt1 : halt0) return;
-  const float fast = step_piece<kBC>(P, cur, cur ? dt1 : dt0, cur ? t1 : t0, pc.bx, pc.row_first,
+  const unsigned dither_step = unsigned(P.it_count0 + (cur ? n1 : n0)) * kDitherStep;
+  const float fast = step_piece<kBC>(P, cur, cur ? dt1 : dt0, cur ? t1 : t0, dither_step, pc.bx, pc.row_first,
                                      pc.row_last, lane);
   publish_min_f32(P.ctrl + cur, fast, P.cn_dx_f, lane);
 }
