@@ -59,9 +59,10 @@ __device__ __forceinline__ float two_sum(float a, float b, float& lo) {
   return s;
 }
 
-// The reference's h < 0 -> 0 on the two-float depth hs + he.
+// The reference's h < 0 -> 0 on the two-float depth hs + he (hs == 0 with
+// he < 0 is left to the final clamp of the rounded depth).
 __device__ __forceinline__ void clamp_neg(float& hs, float& he) {
-  const bool neg = hs < 0.f || (hs == 0.f && he < 0.f);
+  const bool neg = hs < 0.f;
   hs = neg ? 0.f : hs;
   he = neg ? 0.f : he;
 }
@@ -72,22 +73,37 @@ __device__ __forceinline__ void clamp_neg(float& hs, float& he) {
 // of the 1024^2 Thacker bowl, +8.6e-6 at 512^2, linear in the step count, the
 // same with every dry threshold and with IEEE division, square root and no
 // FMA contraction; one FP32 step from an FP64 state showed no such loss, one
-// from an FP32-evolved state did.  Rounded stochastically: -2.9e-7 and
-// +2.3e-7.  hs + he (|he| <= ulp(hs)) goes to the float one ulp away in he's
-// direction with probability |he| / ulp, else stays hs, so the expected
-// stored depth is hs + he; at hs = 2^k the step down is two spacings at half
-// the probability, the same expected value.  u in [0, 1) is a per-cell Weyl
-// sequence over the iterations (golden-ratio increment), so each cell's
-// decisions are equidistributed in time; it depends only on the member's
-// cell and iteration count, so a member's bits do not depend on the batch or
-// the partition, and a hotstart continues the same sequence.
-__device__ __forceinline__ float sround(float hs, float he, float u) {
-  const float ulp = __int_as_float(__float_as_int(hs) & 0x7f800000) * 0x1p-23f;
-  return u * ulp < fabsf(he) ? hs + copysignf(ulp, he) : hs;
+// from an FP32-evolved state did.  Rounded stochastically: +1.2e-9 and
+// -2.9e-9.  The stored depth is hs + he plus a dither uniform over one ulp of
+// hs, rounded to nearest: it lands on the float one ulp away in he's
+// direction with probability |he| / ulp (|he| <= ulp/2), else on hs, so its
+// expected value is hs + he (a dither as wide as a grid spacing keeps this
+// where the spacing halves below a power of two).  The dither is x 2^-32 ulp,
+// x = (column term) + fmix32(row and iteration term): hashed, so a cell's
+// dither at the next step is unrelated to its neighbours' dithers now (with
+// a dither linear in column, row and step, a Weyl sequence, the loss was
+// still -5.6e-6: the neighbours' roundings feed the cell's next residual).
+// It depends only on the member's cell and iteration count, so a member's
+// bits do not depend on the batch or the partition, and a hotstart continues
+// the same sequence.  (Below 2^-71 m the scaled ulp underflows and the depth
+// is rounded to nearest.)
+__device__ __forceinline__ float sround(float hs, float he, int x) {
+  const float ulp_2m32 = __int_as_float(__float_as_int(hs) & 0x7f800000) * 0x1p-55f;
+  return fmaxf(hs + fmaf(float(x), ulp_2m32, he), 0.f);
 }
 constexpr unsigned kDitherCol = 0x85EBCA77u;
 constexpr unsigned kDitherRow = 0xC2B2AE3Du;
 constexpr unsigned kDitherStep = 0x9E3779B9u;  // 2^32 / golden ratio
+// MurmurHash3's 32-bit finaliser: the row-and-iteration part of the dither
+// (wave-uniform, so it runs on the scalar unit)
+__device__ __forceinline__ unsigned fmix32(unsigned h) {
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
 
 struct Side {
   float h, qn, qt, un, ut, sq;
@@ -432,7 +448,7 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
         nqx = low ? 0.f : nqx;
         nqy = low ? 0.f : nqy;
       }
-      nh = sround(hs, he, float(dither_col + (dither_step + unsigned(ru) * kDitherRow)) * 0x1p-32f);
+      nh = sround(hs, he, int(dither_col + fmix32(dither_step + unsigned(ru) * kDitherRow)));
     }
     // x face (c, c+1) of row r: back side stays in this lane (rhs*0), the
     // front side goes to lane c+1 (rhs*1).  Faces outside the reference's

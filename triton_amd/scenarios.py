@@ -135,8 +135,9 @@ def triangular(t_h, t0, tp, t1, peak):
 def watershed(n: int = 4096, seed: int = 3, dx: float = 10.0, hours: float = 6.0,
               rain_peak_mmh: float = 50.0, inflow_peaks=(100.0, 200.0, 350.0, 500.0),
               manning_scale: float = 1.0, courant: float = 0.5, hextra: float = 1e-3,
-              inflow_seed: int = 333, track_maxh: bool = False) -> Case:
-    """Synthetic watershed: n x n cells at dx, dry start, open boundaries.
+              inflow_seed: int = 333, track_maxh: bool = False, open_boundaries: int = 1) -> Case:
+    """Synthetic watershed: n x n cells at dx, dry start, open boundaries
+    (walls with open_boundaries=0).
 
     Rain: one uniform zone, rows every 60 s, triangular 0 -> peak at 1 h -> 0
     at 2 h, then dry (applied as the reference's left-continuous staircase).
@@ -160,7 +161,8 @@ def watershed(n: int = 4096, seed: int = 3, dx: float = 10.0, hours: float = 6.0
         sources = dict(rows=list(pick // n), cols=list(pick % n), time_s=t_hyg_h * 3600.0, q=q)
     return Case.from_arrays(dem=bed, h=np.zeros((n, n)), nman=nman, dx=dx, courant=courant,
                             hextra=hextra, sim_duration=hours * 3600.0, print_interval=hours * 3600.0,
-                            open_boundaries=1, runoff=runoff, sources=sources, track_maxh=track_maxh)
+                            open_boundaries=open_boundaries, runoff=runoff, sources=sources,
+                            track_maxh=track_maxh)
 
 
 def ensemble_sizes(count: int = 64, seed: int = 4, lo_exp: float = 8.0, hi_exp: float = 11.0):
