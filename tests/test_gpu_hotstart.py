@@ -1,6 +1,9 @@
 """Restart equivalence: stopping at a print, writing its BIN rasters and
 starting again from them (checkpoint_id, src/triton_init.h:303-353) gives
-the same steps and the same bits as one uninterrupted run."""
+the same steps and the same bits as one uninterrupted run.  The restart's
+.cfg carries the iteration count as the reference's hotstart .cfg does
+(src/output.h:755-791); the FP32 form keys the dither of its depth rounding
+on it, so it restarts bit for bit too."""
 import re
 import shutil
 
@@ -12,7 +15,7 @@ import triton_amd as ta
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("form", [ta.FORM_F64_FUSED, ta.FORM_F64_UNFUSED])
+@pytest.mark.parametrize("form", [ta.FORM_F64_FUSED, ta.FORM_F64_UNFUSED, ta.FORM_F32_FUSED])
 def test_hotstart_equals_one_run(gpu, ref_tree, tmp_path, form):
     cfg = ref_tree / "test" / "paraboloid.cfg"
     case = ta.Case.load(cfg, ref_tree / "test")
@@ -30,6 +33,7 @@ def test_hotstart_equals_one_run(gpu, ref_tree, tmp_path, form):
     text = cfg.read_text()
     text = re.sub(r"(?m)^checkpoint_id=.*$", "checkpoint_id=6", text)
     text = re.sub(r"(?m)^sim_start_time=.*$", f"sim_start_time={mid['simtime']!r}", text)
+    text = re.sub(r"(?m)^it_count=.*$", f"it_count={mid['it_count']}", text)
     text = re.sub(r"(?m)^output_folder=.*$", f"output_folder={tmp_path / 'out'}", text)
     (hot_dir / "hot.cfg").write_text(text)
     ta.write_rasters(half, case, tmp_path / "out", 6)  # the print-6 rasters, as the run writes them
