@@ -284,6 +284,85 @@ int32_t tr_batch_work_waves(const tr_batch *b);
 void *tr_batch_stream(tr_batch *b);
 void tr_batch_destroy(tr_batch *b);
 
+/* ---- serving: continuous batching over one tr_batch ------------------------ */
+/* Flood-scenario requests (cases run to their sim_duration in the FP32 form)
+ * wait in arrival order; each scheduling round (tr_serve_step) admits waiting
+ * requests first fit while their device bytes fit the HBM budget, runs
+ * chunk_steps batched launches for every admitted request, and retires the
+ * finished ones (final status and, with keep_state, state kept on the host;
+ * member removed, its memory freed), so requests join while others are
+ * mid-run.  A request's result is bit for bit the result it has alone.  The
+ * reference has no serving layer (it runs ensemble members as separate MPI
+ * jobs, src/Ensify.h:43-72); this is the request loop of the batched form.
+ * Not thread-safe: one host thread drives a tr_serve. */
+typedef struct tr_serve tr_serve;
+typedef struct tr_serve_options {
+  int64_t hbm_budget_bytes; /* device bytes the admitted requests may hold (0: no limit) */
+  int32_t chunk_steps;      /* batched launches per scheduling round (>= 1) */
+  int32_t waves_per_simd;   /* launch shape of the batch (0: default) */
+  int32_t max_members;      /* requests on the device at once (0: 256) */
+  int32_t starve_rounds;    /* a waiting request passed over more than this many
+                               rounds holds back every later one until it fits */
+  int32_t keep_state;       /* keep each finished request's final h, qx, qy */
+  int32_t reserved;
+} tr_serve_options;
+
+enum { TR_REQ_WAITING = 0, TR_REQ_RUNNING = 1, TR_REQ_FINISHED = 2 };
+typedef struct tr_serve_request {
+  int32_t state;            /* TR_REQ_* */
+  int32_t member;           /* batch member while running, else -1 */
+  int64_t bytes;            /* device bytes (tr_batch_member_bytes) */
+  int64_t cells;            /* interior cells */
+  int32_t skipped;          /* admission rounds it was passed over */
+  int32_t reserved;
+  double arrival;           /* tr_serve_clock() seconds */
+  double admitted;          /* -1 until admitted */
+  double done;              /* -1 until finished */
+  tr_sim_status status;     /* final status once finished */
+} tr_serve_request;
+
+typedef struct tr_serve_stats {
+  int64_t submitted, completed, waiting, running;
+  int64_t cell_steps;       /* interior cells x steps of the retired requests */
+  int64_t rounds;           /* rounds that ran a chunk */
+  int64_t peak_bytes;       /* most device bytes held at once */
+  int32_t max_members;      /* most requests on the device at once */
+  int32_t reserved;
+  double busy_s;            /* wall time of the rounds that ran a chunk */
+  double p50_latency_s;     /* submit-to-finish latency over retired requests */
+  double p99_latency_s;     /* (linear interpolation; NaN while none) */
+  double max_latency_s;
+} tr_serve_stats;
+
+/* The server's clock (seconds, monotonic): arrival, admitted and done times. */
+double tr_serve_clock(void);
+int tr_serve_create(const tr_serve_options *opt, tr_serve **out);
+/* Queue a request arriving at `arrival` (tr_serve_clock seconds; < 0: now).
+ * The view's arrays must stay valid until the request is admitted
+ * (tr_serve_poll state != TR_REQ_WAITING); TR_ERR_NOMEM when it needs more
+ * than the whole budget. */
+int tr_serve_submit(tr_serve *s, const tr_case_view *v, double arrival, int32_t *request_id);
+/* One scheduling round: admit, run one chunk, retire.  *ran = 0 when no
+ * request was on the device (nothing ran). */
+int tr_serve_step(tr_serve *s, int32_t *ran);
+/* Rounds until every submitted request has finished (TR_ERR_STATE if a
+ * waiting request can never be admitted). */
+int tr_serve_drain(tr_serve *s);
+int tr_serve_poll(const tr_serve *s, int32_t request_id, tr_serve_request *out);
+/* Final state of a finished request (keep_state), rows x cols each. */
+int tr_serve_result(const tr_serve *s, int32_t request_id, double *h, double *qx, double *qy);
+/* Forget a finished request (its kept state too). */
+int tr_serve_release(tr_serve *s, int32_t request_id);
+int tr_serve_stats_get(const tr_serve *s, tr_serve_stats *out);
+void tr_serve_destroy(tr_serve *s);
+/* The admission policy of a round, on its own (host only): walking the
+ * waiting requests in arrival order, request k is admitted (admit[k] = 1)
+ * when held + bytes[k] fits the budget and fewer than max_members run; a
+ * request that does not fit has skipped[k] incremented, and once some
+ * skipped[k] exceeds starve_rounds no later request is admitted this round. */
+int tr_serve_admit(const int64_t *bytes, int32_t *skipped, int32_t n_waiting, int64_t held, int64_t budget,
+                   int32_t running, int32_t max_members, int32_t starve_rounds, uint8_t *admit);
+
 #ifdef __cplusplus
 }
 #endif

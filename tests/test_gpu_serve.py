@@ -64,3 +64,17 @@ def test_mass_of_a_served_request(gpu):
     v0 = float(c.arrays["h"][1:-1, 1:-1].sum()) * dx * dx
     r = alone(c)
     assert abs(r.status["volume"] - v0) / v0 < 1e-6
+
+
+def test_oversized_request_rejected_at_submit(gpu):
+    """A request needing more than the whole HBM budget is refused at submit
+    (TR_ERR_NOMEM) and leaves the server usable."""
+    small = scenarios.circular_dambreak(n=48, radius=8.0, t_end=1.0)
+    big = scenarios.circular_dambreak(n=96, radius=12.0, t_end=1.0)
+    with Server(hbm_budget_bytes=ta.Batch.member_bytes(small) + 1024, chunk_steps=8) as srv:
+        with pytest.raises(ta.TritonError) as e:
+            srv.submit(big)
+        assert e.value.code == ta.TR_ERR_NOMEM
+        rid = srv.submit(small)
+        srv.drain()
+        assert srv.result(rid).status["finished"] and srv.stats.completed == 1

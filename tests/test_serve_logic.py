@@ -1,129 +1,73 @@
-"""Scheduling logic of the serving loop on the CPU, against a stand-in for
-the batch (no device): admission within the HBM budget, arrival order with
-first fit, the starvation guard, retirement, and latency bookkeeping."""
-import pytest
+"""Admission policy of the serving loop on the CPU (tr_serve_admit through the
+C ABI, no device): first fit in arrival order within the HBM budget and the
+member cap, the starvation guard, and, over many rounds of a model of the
+loop that retires requests after a number of rounds, a budget that is never
+exceeded and requests that all finish without starving."""
+import numpy as np
 
-from triton_amd.serve import Server
-
-
-class FakeCase:
-    def __init__(self, side, nbytes, steps):
-        self.rows = self.cols = side + 2
-        self.nbytes = nbytes
-        self.need = steps
-
-
-class FakeBatch:
-    """Steps every member by the chunk; a member finishes after its
-    `need` steps.  Mirrors the Batch methods the server calls."""
-
-    def __init__(self):
-        self.m = {}
-        self.next = 0
-        self.log = []
-
-    @staticmethod
-    def member_bytes(case):
-        return case.nbytes
-
-    @property
-    def device_bytes(self):
-        return sum(c.nbytes for c, _ in self.m.values())
-
-    def add(self, case):
-        mid = self.next
-        self.next += 1
-        self.m[mid] = [case, 0]
-        self.log.append(("add", mid))
-        return mid
-
-    def remove(self, mid):
-        del self.m[mid]
-        self.log.append(("remove", mid))
-
-    def arm(self, max_steps):
-        self.limit = max_steps
-
-    def enqueue(self, n):
-        for rec in self.m.values():
-            rec[1] = min(rec[0].need, rec[1] + min(n, self.limit))
-
-    def sync(self):
-        return sum(1 for c, s in self.m.values() if s < c.need)
-
-    def poll(self, mid):
-        c, s = self.m[mid]
-        return {"finished": s >= c.need, "steps": s}
-
-    def status(self, mid):
-        return self.poll(mid)
-
-    def state(self, mid):
-        return {"steps": self.m[mid][1]}
-
-    def close(self):
-        pass
-
-
-def test_budget_is_never_exceeded_and_all_requests_finish():
-    fb = FakeBatch()
-    srv = Server(hbm_budget_bytes=100, chunk_steps=10, batch=fb)
-    sizes = [60, 30, 50, 10, 40, 70]
-    for i, b in enumerate(sizes):
-        srv.submit(FakeCase(8 + i, b, 25 + 10 * i))
-    peak = 0
-    while srv.pending():
-        srv.step()
-        peak = max(peak, fb.device_bytes)
-        assert srv.stats.peak_bytes <= 100
-    assert peak <= 100
-    assert srv.stats.completed == len(sizes)
-    for rid in range(len(sizes)):
-        r = srv.result(rid)
-        assert r.steps == 25 + 10 * rid
-        assert r.latency is not None and r.latency >= 0
-        assert r.admitted >= r.arrival and r.done >= r.admitted
-    assert srv.stats.cell_steps == sum((8 + i) ** 2 * (25 + 10 * i) for i in range(len(sizes)))
+from triton_amd.serve import admit_round
 
 
 def test_first_fit_in_arrival_order():
-    fb = FakeBatch()
-    srv = Server(hbm_budget_bytes=100, chunk_steps=5, batch=fb)
-    srv.submit(FakeCase(4, 70, 50))   # 0: admitted
-    srv.submit(FakeCase(4, 50, 50))   # 1: does not fit next to 0
-    srv.submit(FakeCase(4, 20, 50))   # 2: fits, admitted before 1
-    srv.step()
-    assert [e for e in fb.log if e[0] == "add"] == [("add", 0), ("add", 1)]
-    admitted = {r.rid for r in srv.running.values()}
-    assert admitted == {0, 2}
-    assert [r.rid for r in srv.waiting] == [1]
+    skipped = np.zeros(3, np.int32)
+    take = admit_round([70, 50, 20], skipped, held=0, budget=100, running=0)
+    assert list(take) == [True, False, True]       # 1 does not fit next to 0; 2 does
+    assert list(skipped) == [0, 1, 0]
+
+
+def test_held_bytes_and_member_cap_count():
+    skipped = np.zeros(3, np.int32)
+    assert list(admit_round([30, 30, 30], skipped, held=50, budget=100, running=0)) == [True, False, False]
+    skipped = np.zeros(3, np.int32)
+    assert list(admit_round([1, 1, 1], skipped, held=0, budget=0, running=2, max_members=4)) == [True, True, False]
+    assert list(skipped) == [0, 0, 1]
+
+
+def test_no_budget_admits_everything():
+    skipped = np.zeros(4, np.int32)
+    assert admit_round([10 ** 12] * 4, skipped, held=0, budget=0, running=0).all()
 
 
 def test_starved_request_holds_back_later_ones():
-    fb = FakeBatch()
-    srv = Server(hbm_budget_bytes=100, chunk_steps=1, batch=fb, starve_rounds=2)
-    srv.submit(FakeCase(4, 90, 1000))  # 0: long-running, holds the budget
-    srv.step()
-    srv.submit(FakeCase(4, 60, 5))     # 1: never fits while 0 runs
-    for _ in range(3):
-        srv.step()                     # 1 passed over three times: now starved
-    srv.submit(FakeCase(4, 10, 5))     # 2: would fit, but 1 is starving
-    srv.step()
-    assert {r.rid for r in srv.running.values()} == {0}
-    assert [r.rid for r in srv.waiting] == [1, 2]
+    skipped = np.array([2, 0], np.int32)           # request 0 already passed over twice
+    take = admit_round([60, 10], skipped, held=90, budget=100, running=1, starve_rounds=2)
+    assert list(take) == [False, False]            # 10 B would fit, but 0 is now starving
+    assert list(skipped) == [3, 0]
+    skipped = np.array([1, 0], np.int32)           # not starving yet: 1 goes ahead
+    take = admit_round([60, 10], skipped, held=90, budget=100, running=1, starve_rounds=2)
+    assert list(take) == [False, True]
 
 
-def test_oversized_request_rejected_at_submit():
-    srv = Server(hbm_budget_bytes=100, batch=FakeBatch())
-    with pytest.raises(ValueError):
-        srv.submit(FakeCase(4, 101, 5))
-
-
-def test_drain_finishes_everything():
-    fb = FakeBatch()
-    srv = Server(hbm_budget_bytes=0, chunk_steps=7, batch=fb)
-    for i in range(5):
-        srv.submit(FakeCase(4, 10, 3 + 11 * i))
-    srv.drain()
-    assert srv.stats.completed == 5 and not fb.m
-    assert srv.stats.percentile(50) is not None and srv.stats.percentile(99) >= srv.stats.percentile(50)
+def test_rounds_keep_the_budget_and_finish_everything():
+    """The loop's rounds with the policy: requests hold their bytes for a
+    number of rounds, then retire; the held bytes never exceed the budget and
+    every request finishes, the large one too (the guard stops smaller later
+    arrivals from starving it)."""
+    rng = np.random.default_rng(7)
+    budget = 100
+    arrivals = [(int(b), int(n)) for b, n in zip(rng.integers(5, 60, 40), rng.integers(1, 6, 40))]
+    arrivals.insert(3, (95, 2))                    # needs nearly the whole budget
+    waiting, running, done, held_max = [], [], [], 0
+    k = 0
+    for _ in range(10000):
+        if k < len(arrivals):                      # one arrival per round
+            waiting.append([k, *arrivals[k], 0])
+            k += 1
+        if not waiting and not running and k == len(arrivals):
+            break
+        skipped = np.array([w[3] for w in waiting], np.int32)
+        held = sum(r[1] for r in running)
+        take = admit_round([w[1] for w in waiting], skipped, held=held, budget=budget, running=len(running),
+                           starve_rounds=4)
+        keep = []
+        for w, t, s in zip(waiting, take, skipped):
+            w[3] = int(s)
+            (running if t else keep).append(w)
+        waiting = keep
+        held_max = max(held_max, sum(r[1] for r in running))
+        for r in running:
+            r[2] -= 1
+        done += [r[0] for r in running if r[2] <= 0]
+        running = [r for r in running if r[2] > 0]
+    assert held_max <= budget
+    assert sorted(done) == list(range(len(arrivals)))

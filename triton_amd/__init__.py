@@ -25,6 +25,7 @@ FORM_F64_FUSED = 1
 FORM_F32_FUSED = 2
 
 TR_OK = 0
+TR_ERR_INVALID, TR_ERR_INPUT, TR_ERR_HIP, TR_ERR_NOMEM, TR_ERR_STATE = 1, 2, 3, 4, 5
 _ERRNAMES = {1: "invalid argument", 2: "input error", 3: "HIP error", 4: "out of memory",
              5: "bad state"}
 

@@ -26,7 +26,7 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["kernels_ref.hip", "step_f64.hip", "step_f32.hip", "batch.hip", "sim.hip"]
-HOST_SOURCES = ["host/case.cpp", "host/terrain.cpp", "capi.cpp"]
+HOST_SOURCES = ["host/case.cpp", "host/terrain.cpp", "host/serve.cpp", "capi.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -1,12 +1,14 @@
 """Serving loop: flood-scenario requests stepped together on one GPU.
 
-Requests (a triton_amd.Case each, run to its sim_duration in the FP32 form)
-are admitted into one tr_batch while their device bytes fit the HBM budget;
-admitted requests keep their state on the device and advance together, one
-batched launch per step for all of them, each with its own dt.  Between
-chunks of launches the loop retires finished requests (downloads their final
-state, frees their memory) and admits waiting ones in arrival order, so new
-requests join while others are mid-run (continuous batching).
+A thin wrapper over the C++ request loop (tr_serve_*, triton_amd/csrc/host/
+serve.cpp): requests (a triton_amd.Case each, run to its sim_duration in the
+FP32 form) are admitted into one tr_batch while their device bytes fit the
+HBM budget; admitted requests keep their state on the device and advance
+together, one batched launch per step for all of them, each with its own dt.
+Between chunks of launches the loop retires finished requests (final status
+and state kept on the host, device memory freed) and admits waiting ones in
+arrival order, so new requests join while others are mid-run (continuous
+batching).
 
 A member's arithmetic does not depend on which other members share its
 launches, so a request's result is bit-for-bit the result it has alone.
@@ -16,28 +18,96 @@ jobs (src/Ensify.h:43-72).
 """
 from __future__ import annotations
 
+import ctypes as C
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Batch, Case
+from . import Case, SimStatus, check, lib
+
+WAITING, RUNNING, FINISHED = 0, 1, 2
+
+
+class ServeOptions(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("chunk_steps", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("max_members", C.c_int32), ("starve_rounds", C.c_int32), ("keep_state", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("state", C.c_int32), ("member", C.c_int32), ("bytes", C.c_int64), ("cells", C.c_int64),
+                ("skipped", C.c_int32), ("reserved", C.c_int32), ("arrival", C.c_double),
+                ("admitted", C.c_double), ("done", C.c_double), ("status", SimStatus)]
+
+
+class ServeStatsC(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("waiting", C.c_int64),
+                ("running", C.c_int64), ("cell_steps", C.c_int64), ("rounds", C.c_int64),
+                ("peak_bytes", C.c_int64), ("max_members", C.c_int32), ("reserved", C.c_int32),
+                ("busy_s", C.c_double), ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double),
+                ("max_latency_s", C.c_double)]
+
+
+def _serve_lib() -> C.CDLL:
+    L = lib()
+    if not getattr(L, "_serve_sig", False):
+        vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        from . import CaseView
+        sig = {
+            "tr_serve_clock": (f64, []),
+            "tr_serve_create": (C.c_int, [C.POINTER(ServeOptions), C.POINTER(vp)]),
+            "tr_serve_submit": (C.c_int, [vp, C.POINTER(CaseView), f64, C.POINTER(i32)]),
+            "tr_serve_step": (C.c_int, [vp, C.POINTER(i32)]),
+            "tr_serve_drain": (C.c_int, [vp]),
+            "tr_serve_poll": (C.c_int, [vp, i32, C.POINTER(ServeRequest)]),
+            "tr_serve_result": (C.c_int, [vp, i32] + [C.POINTER(C.c_double)] * 3),
+            "tr_serve_release": (C.c_int, [vp, i32]),
+            "tr_serve_stats_get": (C.c_int, [vp, C.POINTER(ServeStatsC)]),
+            "tr_serve_destroy": (None, [vp]),
+            "tr_serve_admit": (C.c_int, [C.POINTER(i64), C.POINTER(i32), i32, i64, i64, i32, i32, i32,
+                                         C.POINTER(C.c_uint8)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._serve_sig = True
+    return L
+
+
+def clock() -> float:
+    """The server's clock (seconds): arrival, admitted and done times."""
+    return float(_serve_lib().tr_serve_clock())
+
+
+def admit_round(bytes_, skipped, held: int, budget: int, running: int, max_members: int = 256,
+                starve_rounds: int = 8):
+    """The admission policy of one round (tr_serve_admit, host only): which
+    waiting requests, in arrival order, are admitted; `skipped` (numpy int32)
+    is updated in place."""
+    b = np.ascontiguousarray(bytes_, dtype=np.int64)
+    assert isinstance(skipped, np.ndarray) and skipped.dtype == np.int32 and skipped.flags.c_contiguous
+    take = np.zeros(b.shape[0], np.uint8)
+    check(_serve_lib().tr_serve_admit(b.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      skipped.ctypes.data_as(C.POINTER(C.c_int32)), int(b.shape[0]),
+                                      int(held), int(budget), int(running), int(max_members),
+                                      int(starve_rounds), take.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return take.astype(bool)
 
 
 @dataclass
 class Request:
     rid: int
-    case: Case
-    arrival: float                  # host clock (time.perf_counter) at submission
+    arrival: float
+    admitted: float | None
+    done: float | None
     bytes: int
-    admitted: float | None = None
-    done: float | None = None
-    member: int | None = None
+    cells: int
+    steps: int
+    skipped: int
     status: dict | None = None
     state: dict | None = None
-    cells: int = 0
-    steps: int = 0
-    skipped: int = 0                # admission rounds it was passed over
     tag: object = None
 
     @property
@@ -60,39 +130,44 @@ class ServeStats:
 
 
 class Server:
-    """Continuous-batching request loop over one tr_batch.
+    """Continuous-batching request loop over one tr_batch (tr_serve_*).
 
     hbm_budget_bytes: device bytes the admitted requests may hold together
         (0: no limit).  A request larger than the whole budget is rejected at
         submit.
     chunk_steps: batched launches between scheduling rounds (admission and
         retirement happen between chunks).
-    keep_state: download the final h, qx, qy of each finished request.
-    starve_rounds: a waiting request passed over this many times by smaller
-        later ones holds back everything behind it until it fits.
-    batch: the tr_batch wrapper to drive (default: a new Batch with the
-        budget); anything with the Batch methods used here will do.
+    keep_state: keep the final h, qx, qy of each finished request.
+    starve_rounds: a waiting request passed over more than this many times by
+        smaller later ones holds back everything behind it until it fits.
     """
 
     def __init__(self, hbm_budget_bytes: int = 0, chunk_steps: int = 32, keep_state: bool = True,
-                 waves_per_simd: int = 0, max_members: int = 256, starve_rounds: int = 8, batch=None):
-        if chunk_steps < 1:
-            raise ValueError("chunk_steps must be >= 1")
+                 waves_per_simd: int = 0, max_members: int = 256, starve_rounds: int = 8):
+        opt = ServeOptions(int(hbm_budget_bytes), int(chunk_steps), int(waves_per_simd), int(max_members),
+                           int(starve_rounds), int(bool(keep_state)), 0)
+        L = _serve_lib()
+        h = C.c_void_p()
+        check(L.tr_serve_create(C.byref(opt), C.byref(h)))
+        self._h = h
         self.budget = int(hbm_budget_bytes)
-        self.chunk = int(chunk_steps)
         self.keep_state = keep_state
-        self.max_members = int(max_members)
-        self.starve_rounds = int(starve_rounds)
-        self.batch = batch if batch is not None else Batch(hbm_budget_bytes=self.budget,
-                                                            waves_per_simd=waves_per_simd)
-        self.waiting: list[Request] = []
-        self.running: dict[int, Request] = {}
-        self.finished: dict[int, Request] = {}
-        self.stats = ServeStats()
-        self._next = 0
+        self._cases: dict[int, Case] = {}     # requests not yet retired
+        self._views: dict[int, object] = {}   # their views stay valid until admission
+        self._tags: dict[int, object] = {}
+        self._done: dict[int, Request] = {}
+        self._latencies: list[float] = []
 
     def close(self) -> None:
-        self.batch.close()
+        if getattr(self, "_h", None):
+            _serve_lib().tr_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -102,74 +177,68 @@ class Server:
 
     # -- client side -------------------------------------------------------
     def submit(self, case: Case, tag=None, arrival: float | None = None) -> int:
-        need = self.batch.member_bytes(case)
-        if need < 0:
-            raise ValueError("case cannot run in the batched FP32 form")
-        if self.budget > 0 and need > self.budget:
-            raise ValueError(f"request needs {need} B, more than the whole HBM budget {self.budget} B")
-        rid = self._next
-        self._next += 1
-        self.waiting.append(Request(rid, case, time.perf_counter() if arrival is None else arrival, need,
-                                    cells=(case.rows - 2) * (case.cols - 2), tag=tag))
-        return rid
+        """Queue a request; `arrival` is on the server's clock (default now)."""
+        v = case.view()
+        rid = C.c_int32()
+        check(_serve_lib().tr_serve_submit(self._h, C.byref(v), -1.0 if arrival is None else float(arrival),
+                                           C.byref(rid)))
+        r = rid.value
+        self._cases[r], self._views[r], self._tags[r] = case, v, tag
+        return r
+
+    def poll(self, rid: int) -> ServeRequest:
+        out = ServeRequest()
+        check(_serve_lib().tr_serve_poll(self._h, int(rid), C.byref(out)))
+        return out
+
+    def _stats(self) -> ServeStatsC:
+        st = ServeStatsC()
+        check(_serve_lib().tr_serve_stats_get(self._h, C.byref(st)))
+        return st
 
     def pending(self) -> int:
-        return len(self.waiting) + len(self.running)
+        st = self._stats()
+        return int(st.waiting + st.running)
+
+    @property
+    def running(self) -> list[int]:
+        return [r for r in self._cases if self.poll(r).state == RUNNING]
+
+    @property
+    def waiting(self) -> list[int]:
+        return [r for r in self._cases if self.poll(r).state == WAITING]
 
     # -- scheduler ----------------------------------------------------------
-    def _admit(self) -> None:
-        held = self.batch.device_bytes
-        keep = []
-        blocked = False
-        for req in self.waiting:  # arrival order, first fit
-            fits = (self.budget <= 0 or held + req.bytes <= self.budget) and \
-                len(self.running) < self.max_members
-            if blocked or not fits:
-                if not fits:
-                    req.skipped += 1
-                    blocked = blocked or req.skipped > self.starve_rounds
-                keep.append(req)
+    def _collect(self) -> None:
+        """Move finished requests' results to the host side and release them."""
+        L = _serve_lib()
+        for r in list(self._cases):
+            p = self.poll(r)
+            if p.state == RUNNING:
+                self._views.pop(r, None)
+            if p.state != FINISHED:
                 continue
-            req.member = self.batch.add(req.case)
-            req.admitted = time.perf_counter()
-            held = self.batch.device_bytes
-            self.running[req.member] = req
-        self.waiting = keep
-        self.stats.max_members = max(self.stats.max_members, len(self.running))
-        self.stats.peak_bytes = max(self.stats.peak_bytes, held)
-
-    def _retire(self) -> None:
-        for mid in list(self.running):
-            st = self.batch.poll(mid)
-            if not st["finished"]:
-                continue
-            req = self.running.pop(mid)
-            req.status = self.batch.status(mid)
+            case = self._cases.pop(r)
+            self._views.pop(r, None)
+            state = None
             if self.keep_state:
-                req.state = self.batch.state(mid)
-            req.steps = int(req.status["steps"])
-            req.done = time.perf_counter()
-            self.batch.remove(mid)
-            req.member = None
-            self.finished[req.rid] = req
-            self.stats.completed += 1
-            self.stats.cell_steps += req.cells * req.steps
-            self.stats.latencies.append(req.latency)
+                shape = (case.rows, case.cols)
+                state = {k: np.empty(shape, np.float64) for k in ("h", "qx", "qy")}
+                P = C.POINTER(C.c_double)
+                check(L.tr_serve_result(self._h, int(r), *(state[k].ctypes.data_as(P) for k in ("h", "qx", "qy"))))
+            status = p.status.as_dict()
+            self._done[r] = Request(r, p.arrival, p.admitted, p.done, int(p.bytes), int(p.cells),
+                                    int(status["steps"]), int(p.skipped), status, state, self._tags.pop(r, None))
+            self._latencies.append(p.done - p.arrival)
+            check(L.tr_serve_release(self._h, int(r)))
 
     def step(self) -> bool:
         """One scheduling round: admit, run one chunk, retire.  Returns False
         when nothing was on the device."""
-        self._admit()
-        if not self.running:
-            return False
-        t0 = time.perf_counter()
-        self.batch.arm(max_steps=self.chunk)
-        self.batch.enqueue(self.chunk)
-        self.batch.sync()
-        self._retire()
-        self.stats.busy_s += time.perf_counter() - t0
-        self.stats.rounds += 1
-        return True
+        ran = C.c_int32()
+        check(_serve_lib().tr_serve_step(self._h, C.byref(ran)))
+        self._collect()
+        return bool(ran.value)
 
     def drain(self) -> None:
         """Run until every submitted request has finished."""
@@ -178,7 +247,13 @@ class Server:
                 raise RuntimeError("waiting requests cannot be admitted")
 
     def result(self, rid: int) -> Request:
-        return self.finished[rid]
+        return self._done[rid]
+
+    @property
+    def stats(self) -> ServeStats:
+        st = self._stats()
+        return ServeStats(int(st.completed), int(st.cell_steps), float(st.busy_s), int(st.rounds),
+                          int(st.max_members), int(st.peak_bytes), list(self._latencies))
 
 
 def serve_poisson(requests, rate_per_s: float, seed: int = 5, hbm_budget_bytes: int = 0,
@@ -191,16 +266,16 @@ def serve_poisson(requests, rate_per_s: float, seed: int = 5, hbm_budget_bytes: 
     offsets = np.cumsum(rng.exponential(1.0 / rate_per_s, size=len(requests)))
     offsets -= offsets[0]
     srv = Server(hbm_budget_bytes, chunk_steps, keep_state=keep_state)
-    t0 = time.perf_counter()
+    t0 = clock()
     k = 0
     while k < len(requests) or srv.pending():
-        now = time.perf_counter() - t0
+        now = clock() - t0
         while k < len(requests) and offsets[k] <= now:
             srv.submit(requests[k], tag=k, arrival=t0 + offsets[k])
             k += 1
             if log:
                 log(f"t={now:7.2f}s submit {k - 1} ({requests[k - 1].rows - 2}^2), "
-                    f"running {len(srv.running)}, waiting {len(srv.waiting)}")
+                    f"pending {srv.pending()}")
         if not srv.step() and k < len(requests):
-            time.sleep(max(0.0, min(0.05, offsets[k] - (time.perf_counter() - t0))))
+            time.sleep(max(0.0, min(0.05, offsets[k] - (clock() - t0))))
     return srv, offsets
