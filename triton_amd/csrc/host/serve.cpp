@@ -118,16 +118,21 @@ int admit(tr_serve* s) {
                           s->opt.hbm_budget_bytes, int32_t(s->running.size()), s->opt.max_members,
                           s->opt.starve_rounds, take.data());
   if (rc) return rc;
+  // a failed add leaves that request and the ones after it waiting, so the
+  // queues stay consistent when the error is returned
   std::deque<int32_t> keep;
   for (int32_t k = 0; k < n; k++) {
     Request& r = s->req.at(s->waiting[size_t(k)]);
     r.info.skipped = skipped[size_t(k)];
-    if (!take[size_t(k)]) {
+    if (rc || !take[size_t(k)]) {
       keep.push_back(r.id);
       continue;
     }
     int32_t mid = -1;
-    if ((rc = tr_batch_add(s->batch, &r.view, &mid))) return rc;
+    if ((rc = tr_batch_add(s->batch, &r.view, &mid))) {
+      keep.push_back(r.id);
+      continue;
+    }
     r.info.state = TR_REQ_RUNNING;
     r.info.member = mid;
     r.info.admitted = now_s();
@@ -137,7 +142,7 @@ int admit(tr_serve* s) {
   s->waiting.swap(keep);
   s->stats.max_members = std::max(s->stats.max_members, int32_t(s->running.size()));
   s->stats.peak_bytes = std::max(s->stats.peak_bytes, tr_batch_device_bytes(s->batch));
-  return TR_OK;
+  return rc;
 }
 
 }  // namespace
