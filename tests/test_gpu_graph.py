@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import triton_amd as ta
+from triton_amd.tune import DEFAULT_CANDIDATES
 from triton_amd import scenarios
 from test_gpu_sim import synthetic
 
@@ -64,11 +65,11 @@ def test_autotune_changes_speed_only(gpu, tmp_path, monkeypatch):
             ids = [b.add(c) for c in members()]
             if tune:
                 choice = b.autotune(steps=4)
-                assert choice in [(5, 0), (10, 0), (15, 0)]
+                assert choice in DEFAULT_CANDIDATES
                 assert (tmp_path / "tune.json").exists()
                 done = b.poll(ids[0])["steps"]
             else:
-                done = 2 * 4 * 3  # what the tuner takes: 2 x steps per candidate
+                done = 2 * 4 * len(DEFAULT_CANDIDATES)  # what the tuner takes: 2 x steps per candidate
                 b.arm(max_steps=done)
                 b.enqueue(done)
                 b.sync()
@@ -76,7 +77,7 @@ def test_autotune_changes_speed_only(gpu, tmp_path, monkeypatch):
             b.enqueue(40)
             b.sync()
             res.append([(b.poll(m)["steps"], b.state(m)["h"]) for m in ids])
-            assert done == 24
+            assert done == 2 * 4 * len(DEFAULT_CANDIDATES)
     for (sa, ha), (sb, hb) in zip(*res):
         assert sa == sb
         np.testing.assert_array_equal(ha, hb)

@@ -399,7 +399,7 @@ Member* member(tr_batch* b, int id) {
 extern "C" {
 
 int tr_batch_create(int64_t hbm_budget_bytes, int32_t waves_per_simd, tr_batch** out) {
-  if (!out || hbm_budget_bytes < 0 || waves_per_simd < 0 || waves_per_simd > 16) {
+  if (!out || hbm_budget_bytes < 0 || waves_per_simd < 0 || waves_per_simd > 32) {
     set_error("tr_batch_create: bad arguments");
     return TR_ERR_INVALID;
   }
@@ -747,7 +747,7 @@ int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
 // steps per captured graph (0: eager launches).  Results do not depend on
 // either.
 int tr_batch_set_launch(tr_batch* b, int32_t waves_per_simd, int32_t graph_steps) {
-  if (!b || waves_per_simd < 0 || waves_per_simd > 16 || graph_steps < 0 || graph_steps > 4096) {
+  if (!b || waves_per_simd < 0 || waves_per_simd > 32 || graph_steps < 0 || graph_steps > 4096) {
     set_error("tr_batch_set_launch: bad arguments");
     return TR_ERR_INVALID;
   }

@@ -14,12 +14,14 @@ import math
 import os
 from pathlib import Path
 
-# (waves per SIMD, steps per graph).  Up to 5 waves per SIMD are resident at
-# once (the step kernel's registers); whole multiples of 5 queue behind them,
-# which evens out uneven pieces (an ensemble of mixed sizes) but costs L2
-# sharing between neighbouring bands when the pieces are equal (one large
-# grid).
-DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0))
+# (waves per SIMD, steps per graph).  About 5 waves per SIMD are resident at
+# once (the step kernel's registers); more queue behind them, which evens out
+# uneven pieces (an ensemble of mixed sizes: on the 64-member ensemble 10,
+# 15, 20, 25 and 30 waves per SIMD measured 94.8, 98.5, 100.5, 100.4 and
+# 99.8 GCUPS) but costs L2 sharing between neighbouring bands when the
+# pieces are equal (one large grid, where 5 is fastest).  hipGraph replays
+# measured the same as eager launches, so the candidates do not use them.
+DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0), (20, 0), (25, 0))
 
 
 def cache_path() -> Path:
