@@ -249,13 +249,14 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
             "bytes_per_cell": b_alg(False, True, True)}
 
 
-def serving(args, capacity_gcups):
+def serving(args, capacity_gcups, waves_per_simd=0):
     """Serving leg: Poisson arrivals (seed 5) of watershed requests, sides
     round(2^U(8,12)), horizons U(1,6) h, request seeds 5000+k, rain as in the
     watershed generator; continuous batching under an HBM budget.  The rate
     is set for the requested offered load from an a-priori work estimate of
     each request (interior cells x horizon / 1.2 s, the CFL dt of 1 m deep
-    water moving at 1 m/s on the 10 m grid)."""
+    water moving at 1 m/s on the 10 m grid).  The batch takes the launch
+    shape (waves per SIMD) the ensemble was tuned to."""
     import numpy as np
     from triton_amd import scenarios
     from triton_amd.serve import serve_poisson
@@ -271,7 +272,8 @@ def serving(args, capacity_gcups):
     rate = args.serve_load * capacity_gcups * 1e9 / est.mean()
     budget = int(args.serve_budget_gb * 2 ** 30)
     t0 = time.perf_counter()
-    srv, offsets = serve_poisson(cases, rate, seed=5, hbm_budget_bytes=budget, chunk_steps=32)
+    srv, offsets = serve_poisson(cases, rate, seed=5, hbm_budget_bytes=budget, chunk_steps=32,
+                                 waves_per_simd=waves_per_simd)
     wall = time.perf_counter() - t0
     st = srv.stats
     out = {"requests": n, "arrival_rate_per_s": round(rate, 4), "offered_load": args.serve_load,
@@ -281,6 +283,7 @@ def serving(args, capacity_gcups):
            "gcups_busy": round(st.cell_steps / max(st.busy_s, 1e-9) / 1e9, 3),
            "wall_s": round(wall, 2), "max_concurrent": st.max_members,
            "peak_gb": round(st.peak_bytes / 2 ** 30, 3), "rounds": st.rounds, "chunk_steps": 32,
+           "waves_per_simd": waves_per_simd,
            "sides_range": [min(sides), max(sides)], "horizon_h_range": [round(min(hours), 2), round(max(hours), 2)]}
     srv.close()
     log(f"serving: p50 {out['p50_latency_s']} s, p99 {out['p99_latency_s']} s, {out['gcups_wall']} GCUPS")
@@ -375,7 +378,7 @@ def main():
         if k in res:
             out[k] = res[k]
     if args.workload == "ensemble" and args.serve_requests > 0:
-        out["serving"] = serving(args, res["value"])
+        out["serving"] = serving(args, res["value"], res["config"]["autotuned_launch"][0])
     print(json.dumps(out), flush=True)
 
 

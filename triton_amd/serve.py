@@ -257,7 +257,7 @@ class Server:
 
 
 def serve_poisson(requests, rate_per_s: float, seed: int = 5, hbm_budget_bytes: int = 0,
-                  chunk_steps: int = 32, keep_state: bool = False, log=None):
+                  chunk_steps: int = 32, keep_state: bool = False, log=None, waves_per_simd: int = 0):
     """Replay `requests` (Cases) arriving as a Poisson process of the given
     rate against a Server; returns (server, arrival offsets).  Arrivals are
     checked between chunks, so a request waits at most one chunk before it
@@ -265,7 +265,7 @@ def serve_poisson(requests, rate_per_s: float, seed: int = 5, hbm_budget_bytes: 
     rng = np.random.default_rng(seed)
     offsets = np.cumsum(rng.exponential(1.0 / rate_per_s, size=len(requests)))
     offsets -= offsets[0]
-    srv = Server(hbm_budget_bytes, chunk_steps, keep_state=keep_state)
+    srv = Server(hbm_budget_bytes, chunk_steps, keep_state=keep_state, waves_per_simd=waves_per_simd)
     t0 = clock()
     k = 0
     while k < len(requests) or srv.pending():
