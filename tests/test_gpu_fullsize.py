@@ -26,7 +26,7 @@ shoreline band, mean |dh| <= 1e-5 m, relative mass <= 1e-6, wet/dry
 mismatches in under 0.1 % of cells, the oracle's simulated time to 1e-8
 relative.  The
 ensemble members run in one batch under the launch the benchmark tunes to
-(15 waves per SIMD, hipGraph replays of 10 steps) and equal, bit for bit,
+(20 waves per SIMD), with hipGraph replays of 10 steps, and equal, bit for bit,
 each member run alone with the default eager launch.
 
 Mass over a whole event: the watershed's 4096^2 terrain, rain and inflows
@@ -138,7 +138,7 @@ def test_ensemble_members_window(gpu):
     print(f"[fullsize] ensemble members {sizes}: generate and spin up to t = 3600 s", flush=True)
     cases = [scenarios.spun_up(c, SPINUP_S) for c in scenarios.ensemble(count=8, sizes=sizes)]
     with ta.Batch() as b:
-        b.set_launch(15, 10)
+        b.set_launch(20, 10)
         ids = [b.add(c) for c in cases]
         b.run(max_steps=WINDOW)
         batched = [(b.status(m), b.state(m)) for m in ids]
