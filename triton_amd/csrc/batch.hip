@@ -623,12 +623,12 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   std::memset(&cp, 0, sizeof(cp));
   StepCtrl& c0 = cp.ctrl[1];
   c0.simtime = v.sim_start;
-  c0.dtmin_bits = ~0ull;
+  reset_dtmin(c0);
   c0.print_id = v.checkpoint_id;
   if (c0.print_id == 0 && c0.simtime > 0.0) c0.print_id = int(c0.simtime / v.print_interval);
   c0.roff_row = v.roff_row0;
   c0.last_obs = v.sim_start;  // src/triton_simulate.h:47
-  cp.ctrl[0].dtmin_bits = ~0ull;
+  reset_dtmin(cp.ctrl[0]);
   cp.cursor = 1;
   RC(put_ctrl(b, id));
   RC(put_inst(b, id));

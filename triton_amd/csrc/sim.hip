@@ -364,7 +364,7 @@ int prime_fused(tr_sim* s) {
   StepCtrl c{};
   c.simtime = v.sim_start;
   c.dt = 0.0;
-  c.dtmin_bits = ~0ull;
+  reset_dtmin(c);
   c.halt = 0;
   c.step = 0;
   c.print_id = v.checkpoint_id;

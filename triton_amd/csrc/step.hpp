@@ -15,10 +15,18 @@ namespace tr {
 // reduces its per-cell dt into that slot; a one-thread close kernel then
 // writes slot (n+1)&1 for the next step (the host-side loop of
 // src/triton_simulate.h:54-134 and :202-226, moved on the device).
+// The step's dt minimum is published into one of kDtParts slots, each on
+// its own 128-B line, by block index modulo 8 (the XCD the block runs on), so
+// the waves of a launch do not all contend for one address; the close
+// kernel reduces the slots.
+constexpr int kDtParts = 8;
+struct alignas(128) DtPart {
+  unsigned long long bits;
+};
+
 struct alignas(128) StepCtrl {
   double simtime;                 // time at the start of this step
   double dt;                      // dt applied in this step (already clamped)
-  unsigned long long dtmin_bits;  // min per-cell dt of this step's result
   int pad2;
   int halt;                       // nonzero: the step is skipped
   long long step;                 // steps completed before this one
@@ -31,7 +39,17 @@ struct alignas(128) StepCtrl {
   double step_vol_in;             // volume this step adds
   double last_obs;                // time of the last observation interval boundary
   double dt_acc;                  // sum of dt since the last it_print boundary
+  DtPart dtmin[kDtParts];         // min per-cell dt of this step's result (bits), per XCD
 };
+
+__host__ __device__ inline void reset_dtmin(StepCtrl& c) {
+  for (int i = 0; i < kDtParts; i++) c.dtmin[i].bits = ~0ull;
+}
+__host__ __device__ inline unsigned long long dtmin_of(const StepCtrl& c) {
+  unsigned long long m = ~0ull;
+  for (int i = 0; i < kDtParts; i++) m = c.dtmin[i].bits < m ? c.dtmin[i].bits : m;
+  return m;
+}
 
 constexpr int kMaxSrcFused = 256;  // point sources handled by the fused step
 

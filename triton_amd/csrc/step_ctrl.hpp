@@ -96,7 +96,7 @@ __device__ void prepare_next(const P& p, const StepCtrl& pv, StepCtrl& nx,
 
   nx.simtime = simtime;
   nx.dt = dt;
-  nx.dtmin_bits = ~0ull;
+  reset_dtmin(nx);
   nx.halt = halt ? 1 : 0;
   nx.step = step;
   nx.print_id = print_id;
@@ -118,19 +118,20 @@ __device__ void carry_halted(const P& p) {
   const StepCtrl& pv = p.ctrl[p.cur];
   StepCtrl& nx = p.ctrl[p.cur ^ 1];
   nx = pv;
-  nx.dtmin_bits = ~0ull;
+  reset_dtmin(nx);
   for (int k = 0; k < p.nsrc; k++)
     p.src_add[(p.cur ^ 1) * p.nsrc + k] = p.src_add[p.cur * p.nsrc + k];
 }
 
-// Lane 0 of a wave publishes its dt min with a relaxed device-scope atomic.
+// Lane 0 of a wave publishes its dt min with a relaxed device-scope atomic
+// into the slot of its block's XCD.
 // The step kernel does nothing else with the control block: the close kernel
 // launched after it (stream order makes every min visible) prepares the next
 // slot.  A grid-wide "last wave" inside the step kernel would need a
 // device-scope fence per wave, which on this multi-die part writes back and
 // invalidates the die's L2 each time.
 __device__ __forceinline__ void publish_min(StepCtrl* cc, unsigned long long wave_min) {
-  if (wave_min != ~0ull) atomicMin(&cc->dtmin_bits, wave_min);
+  if (wave_min != ~0ull) atomicMin(&cc->dtmin[blockIdx.x & (kDtParts - 1)].bits, wave_min);
 }
 
 // Close kernel body for one instance (single thread): prepare the other slot
@@ -139,7 +140,7 @@ __device__ __forceinline__ void publish_min(StepCtrl* cc, unsigned long long wav
 template <class P>
 __device__ void close_one(const P& p, int cur, bool after_step) {
   const StepCtrl& cc = p.ctrl[cur];
-  prepare_next(p, cc, p.ctrl[cur ^ 1], cc.dtmin_bits, after_step, cur ^ 1);
+  prepare_next(p, cc, p.ctrl[cur ^ 1], dtmin_of(cc), after_step, cur ^ 1);
 }
 
 }  // namespace tr
