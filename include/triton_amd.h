@@ -48,6 +48,10 @@ const char *tr_version(void);
 #define TR_FORM_F64_UNFUSED 0 /* reference kernel sequence, dt read back each step */
 #define TR_FORM_F64_FUSED 1   /* one fused launch per step, dt and time on device */
 #define TR_FORM_F32_FUSED 2   /* FP32 depth/discharge, FP64 time and volumes */
+#define TR_FORM_F64_R32 3     /* the FP64 fused step with h, qx, qy rounded to FP32 after
+                                 every update: FP64 arithmetic on FP32-stored state, the
+                                 yardstick for the FP32 form's storage error (no
+                                 counterpart in the reference) */
 
 /* ---- loaded case (input formats) ----------------------------------------- */
 typedef struct tr_case tr_case;
@@ -307,14 +311,16 @@ typedef struct tr_serve_options {
   int32_t reserved;
 } tr_serve_options;
 
-enum { TR_REQ_WAITING = 0, TR_REQ_RUNNING = 1, TR_REQ_FINISHED = 2 };
+/* FAILED: the request could not be admitted (error holds the TR_* code);
+ * CANCELLED: ended by tr_serve_cancel. */
+enum { TR_REQ_WAITING = 0, TR_REQ_RUNNING = 1, TR_REQ_FINISHED = 2, TR_REQ_FAILED = 3, TR_REQ_CANCELLED = 4 };
 typedef struct tr_serve_request {
   int32_t state;            /* TR_REQ_* */
   int32_t member;           /* batch member while running, else -1 */
   int64_t bytes;            /* device bytes (tr_batch_member_bytes) */
   int64_t cells;            /* interior cells */
   int32_t skipped;          /* admission rounds it was passed over */
-  int32_t reserved;
+  int32_t error;            /* TR_* code of a failed request, else TR_OK */
   double arrival;           /* tr_serve_clock() seconds */
   double admitted;          /* -1 until admitted */
   double done;              /* -1 until finished */
@@ -332,6 +338,8 @@ typedef struct tr_serve_stats {
   double p50_latency_s;     /* submit-to-finish latency over retired requests */
   double p99_latency_s;     /* (linear interpolation; NaN while none) */
   double max_latency_s;
+  int64_t failed;           /* requests that could not be admitted */
+  int64_t cancelled;        /* requests ended by tr_serve_cancel */
 } tr_serve_stats;
 
 /* The server's clock (seconds, monotonic): arrival, admitted and done times. */
@@ -339,8 +347,10 @@ double tr_serve_clock(void);
 int tr_serve_create(const tr_serve_options *opt, tr_serve **out);
 /* Queue a request arriving at `arrival` (tr_serve_clock seconds; < 0: now).
  * The view's arrays must stay valid until the request is admitted
- * (tr_serve_poll state != TR_REQ_WAITING); TR_ERR_NOMEM when it needs more
- * than the whole budget. */
+ * (tr_serve_poll state != TR_REQ_WAITING).  The view is checked as
+ * tr_batch_add checks it (TR_ERR_INVALID); TR_ERR_NOMEM when it needs more
+ * than the whole budget.  A request whose admission still fails is ended
+ * with state TR_REQ_FAILED and the others carry on. */
 int tr_serve_submit(tr_serve *s, const tr_case_view *v, double arrival, int32_t *request_id);
 /* One scheduling round: admit, run one chunk, retire.  *ran = 0 when no
  * request was on the device (nothing ran). */
@@ -351,8 +361,11 @@ int tr_serve_drain(tr_serve *s);
 int tr_serve_poll(const tr_serve *s, int32_t request_id, tr_serve_request *out);
 /* Final state of a finished request (keep_state), rows x cols each. */
 int tr_serve_result(const tr_serve *s, int32_t request_id, double *h, double *qx, double *qy);
-/* Forget a finished request (its kept state too). */
+/* Forget an ended (finished, failed or cancelled) request and its kept state. */
 int tr_serve_release(tr_serve *s, int32_t request_id);
+/* End a waiting or running request now (its device memory is freed; the
+ * status of a running one is that of the last round). */
+int tr_serve_cancel(tr_serve *s, int32_t request_id);
 int tr_serve_stats_get(const tr_serve *s, tr_serve_stats *out);
 void tr_serve_destroy(tr_serve *s);
 /* The admission policy of a round, on its own (host only): walking the

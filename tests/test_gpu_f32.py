@@ -160,3 +160,46 @@ def test_unfused_degenerate_shapes_bitwise(gpu, shape):
     assert rs["steps"] == ro["steps"]
     for k in ("h", "qx", "qy"):
         assert np.array_equal(st[k], orc.state[k]), k
+
+
+def test_poll_after_the_control_pool_grows(gpu):
+    """tr_batch_poll reads the host mirror of the control blocks; adding a
+    ninth member grows the pool, and an earlier member's poll still returns
+    what the last sync read (not uninitialised memory)."""
+    with ta.Batch() as b:
+        first = b.add(synthetic(40, 40, seed=31, dur=5.0))
+        b.run(max_steps=7)
+        before = b.poll(first)
+        assert before["steps"] == 7
+        for k in range(9):
+            b.add(synthetic(24, 24, seed=40 + k, dur=1.0))
+        after = b.poll(first)
+        for key in ("steps", "simtime", "dt", "print_id", "volume_in"):
+            assert after[key] == before[key], key
+
+
+def test_f32_friction_below_1e_minus_3(gpu):
+    """Manning n = 5e-4 everywhere (n^2 = 2.5e-7): the FP32 form applies
+    friction per cell at n^2 > 1e-12 as the reference does
+    (src/kernels_state.h:85), so its discharge tracks the oracle's with
+    friction and not the frictionless run's.  Sheet flow 5 cm deep down a 1 %
+    slope, open edges, 20 s."""
+    nr, nc, dx = 40, 80, 1.5
+    y, x = np.mgrid[0:nr, 0:nc]
+    dem = -0.01 * dx * x
+    h = np.full((nr, nc), 0.05)
+    runs = {}
+    for name, n in (("fric", 5e-4), ("none", None)):
+        case = ta.Case.from_arrays(dem=dem, h=h, qx=0.05 * np.ones((nr, nc)), nman=n, dx=dx,
+                                   courant=0.45, hextra=1e-3, sim_duration=20.0, open_boundaries=1)
+        o = Oracle.from_case(case)
+        o.run()
+        runs[name] = o.state["qx"][1:-1, 1:-1].copy()
+        if n is not None:
+            with ta.Simulation(case, ta.FORM_F32_FUSED) as sim:
+                sim.run()
+                got = sim.state()["qx"][1:-1, 1:-1]
+    sep = np.abs(runs["fric"] - runs["none"]).mean()
+    assert sep > 1e-5, "friction too weak to tell the runs apart"
+    err = np.abs(got - runs["fric"]).mean()
+    assert err < 0.01 * sep, (err, sep)

@@ -78,3 +78,48 @@ def test_oversized_request_rejected_at_submit(gpu):
         rid = srv.submit(small)
         srv.drain()
         assert srv.result(rid).status["finished"] and srv.stats.completed == 1
+
+
+def test_invalid_request_refused_at_submit_others_finish(gpu):
+    """A request whose view tr_batch_add would refuse (a runoff zone id out of
+    range) is refused at submit with TR_ERR_INVALID; a valid request submitted
+    beside it still finishes, bit-equal to itself alone."""
+    good = scenarios.watershed(n=64, seed=12, hours=0.05, rain_peak_mmh=500.0, inflow_peaks=())
+    bad = scenarios.watershed(n=64, seed=13, hours=0.05, rain_peak_mmh=500.0, inflow_peaks=())
+    bad.arrays["runoff_id"][5, 7] = 3  # one zone only
+    ref = alone(good)
+    with Server(chunk_steps=8) as srv:
+        with pytest.raises(ta.TritonError) as e:
+            srv.submit(bad)
+        assert e.value.code == ta.TR_ERR_INVALID and "zone" in str(e.value)
+        rid = srv.submit(good)
+        srv.drain()
+        r = srv.result(rid)
+        assert r.status["finished"] and r.steps == ref.steps
+        for k in ("h", "qx", "qy"):
+            np.testing.assert_array_equal(r.state[k], ref.state[k])
+
+
+def test_cancel_waiting_and_running_requests(gpu):
+    """tr_serve_cancel ends a running request (its memory is freed) and a
+    waiting one; the remaining request finishes with its alone result."""
+    cases = [scenarios.circular_dambreak(n=64, radius=10.0, t_end=30.0),
+             scenarios.circular_dambreak(n=72, radius=10.0, t_end=4.0),
+             scenarios.circular_dambreak(n=80, radius=10.0, t_end=30.0)]
+    ref = alone(cases[1])
+    need = [ta.Batch.member_bytes(c) for c in cases]
+    with Server(hbm_budget_bytes=need[0] + need[1] + 1024, chunk_steps=4) as srv:
+        ids = [srv.submit(c) for c in cases]
+        srv.step()
+        assert srv.poll(ids[0]).state == 1 and srv.poll(ids[2]).state == 0
+        held = srv.poll(ids[0]).bytes
+        srv.cancel(ids[0])
+        srv.cancel(ids[2])
+        assert srv.ended(ids[0])[0] == 4 and srv.ended(ids[2])[0] == 4
+        assert srv._stats().cancelled == 2 and held > 0
+        srv.drain()
+        r = srv.result(ids[1])
+        assert r.steps == ref.steps
+        for k in ("h", "qx", "qy"):
+            np.testing.assert_array_equal(r.state[k], ref.state[k])
+        assert srv.stats.completed == 1

@@ -23,6 +23,7 @@ LIB_PATH = PKG / "_lib" / "libtriton_amd.so"
 FORM_F64_UNFUSED = 0
 FORM_F64_FUSED = 1
 FORM_F32_FUSED = 2
+FORM_F64_R32 = 3  # FP64 arithmetic, state rounded to FP32 after every update (accuracy yardstick)
 
 TR_OK = 0
 TR_ERR_INVALID, TR_ERR_INPUT, TR_ERR_HIP, TR_ERR_NOMEM, TR_ERR_STATE = 1, 2, 3, 4, 5

@@ -20,6 +20,7 @@
 
 namespace tr {
 int copy_obs_f32(int size, const float* arr, double* obs, const int* index, hipStream_t s);
+int check_member_view(const tr_case_view* v);
 namespace {
 
 #define RC(x)            \
@@ -166,6 +167,8 @@ int grow(tr_batch* b, int need) {
   TR_HIP(hipHostMalloc(reinterpret_cast<void**>(&hc), sizeof(CtrlPair) * size_t(cap)));
   if (b->cap > 0) {
     TR_HIP(hipMemcpy(dc, b->d_ctrl, sizeof(CtrlPair) * size_t(b->cap), hipMemcpyDeviceToDevice));
+    // the host mirror keeps what the last sync read (tr_batch_poll reads it)
+    std::memcpy(hc, b->h_ctrl, sizeof(CtrlPair) * size_t(b->cap));
   }
   hipFree(b->d_inst);
   hipFree(b->d_ctrl);
@@ -215,41 +218,49 @@ bool uniform_zones(const tr_case_view& v) {
   return true;
 }
 
+// The n^2 array is uploaded when any cell has n^2 > 0; the step decides
+// friction per cell at n^2 > the member's dry threshold (1e-12 by default,
+// the reference's EPS12, src/kernels_state.h:85).
 bool has_friction(const tr_case_view& v) {
   const size_t n = size_t(v.rows) * v.cols;
   for (size_t i = 0; i < n; i++)
-    if (v.nman2[i] > 1e-6) return true;
+    if (v.nman2[i] > 0.0) return true;
   return false;
 }
 
-int check_member_view(const tr_case_view* v) {
+}  // namespace
+
+// Everything tr_batch_add checks of a view before it allocates (also run by
+// tr_serve_submit, so a request that could never be admitted is refused at
+// submit).
+int tr::check_member_view(const tr_case_view* v) {
   if (!v || v->rows < 3 || v->cols < 3 || !v->h || !v->qx || !v->qy || !v->dem || !v->nman2) {
-    set_error("tr_batch_add: view needs rows,cols >= 3 and h, qx, qy, dem, nman2");
+    set_error("member view: view needs rows,cols >= 3 and h, qx, qy, dem, nman2");
     return TR_ERR_INVALID;
   }
   if (int64_t(v->rows) * v->cols > int64_t(INT32_MAX)) {
-    set_error("tr_batch_add: grid exceeds 2^31 cells");
+    set_error("member view: grid exceeds 2^31 cells");
     return TR_ERR_INVALID;
   }
   if (!(v->dx > 0) || !(v->courant > 0) || !(v->print_interval > 0)) {
-    set_error("tr_batch_add: dx, courant and print_interval must be positive");
+    set_error("member view: dx, courant and print_interval must be positive");
     return TR_ERR_INVALID;
   }
   if (v->num_extbc_cells > 0 && (!v->bc_index || !v->bc_type || !v->bc_start || !v->bc_nrows ||
                                   !v->bc_var1 || !v->bc_var2 || v->bc_nvars < 1)) {
-    set_error("tr_batch_add: external boundary segments need their index, type and series arrays");
+    set_error("member view: external boundary segments need their index, type and series arrays");
     return TR_ERR_INVALID;
   }
   if (v->num_runoffs > 0 && (v->roff_nrows < 2 || !v->roff_time || !v->roff_rate)) {
-    set_error("tr_batch_add: runoff needs at least two rows");
+    set_error("member view: runoff needs at least two rows");
     return TR_ERR_INVALID;
   }
   if (v->num_src > 0 && (v->hyg_nrows < 2 || !v->hyg_time || !v->hyg_val || !v->src_pos)) {
-    set_error("tr_batch_add: sources need a hydrograph with at least two rows");
+    set_error("member view: sources need a hydrograph with at least two rows");
     return TR_ERR_INVALID;
   }
   if (v->num_src > kMaxSrcFused) {
-    set_error("tr_batch_add: too many point sources for the fused step");
+    set_error("member view: too many point sources for the fused step");
     return TR_ERR_INVALID;
   }
   const int64_t n = int64_t(v->rows) * v->cols;
@@ -257,7 +268,7 @@ int check_member_view(const tr_case_view* v) {
     int r = v->src_pos[i] / v->cols, c = v->src_pos[i] % v->cols;
     if (v->src_pos[i] < 0 || v->src_pos[i] >= n || r < 1 || r > v->rows - 2 || c < 1 ||
         c > v->cols - 2) {
-      set_error("tr_batch_add: source cell outside the interior");
+      set_error("member view: source cell outside the interior");
       return TR_ERR_INVALID;
     }
   }
@@ -266,13 +277,15 @@ int check_member_view(const tr_case_view* v) {
       for (int c = 1; c < v->cols - 1; c++) {
         int z = v->runoff_id[int64_t(r) * v->cols + c];
         if (z < 0 || z >= v->num_runoffs) {
-          set_error("tr_batch_add: runoff zone id out of range");
+          set_error("member view: runoff zone id out of range");
           return TR_ERR_INVALID;
         }
       }
   }
   return TR_OK;
 }
+
+namespace {
 
 int64_t estimate_bytes(const tr_case_view& v, bool fric, bool zones) {
   const int64_t n = int64_t(v.rows) * v.cols;

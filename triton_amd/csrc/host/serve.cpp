@@ -23,7 +23,9 @@
 
 namespace tr {
 void set_error(const std::string& msg);
+int check_member_view(const tr_case_view* v);
 }
+const char* tr_last_error(void);
 
 namespace {
 
@@ -68,6 +70,18 @@ namespace {
 int fail(int code, const std::string& msg) {
   tr::set_error(msg);
   return code;
+}
+
+// End a request that will not finish: state TR_REQ_FAILED (code = the TR_*
+// error) or TR_REQ_CANCELLED; not counted in the completion statistics.
+void end_request(tr_serve* s, Request& r, int32_t state, int code) {
+  r.info.state = state;
+  r.info.error = code;
+  r.info.member = -1;
+  r.info.done = now_s();
+  r.view = tr_case_view{};
+  if (state == TR_REQ_FAILED) s->stats.failed++;
+  else s->stats.cancelled++;
 }
 
 // Retire every running request whose member has finished.
@@ -118,19 +132,26 @@ int admit(tr_serve* s) {
                           s->opt.hbm_budget_bytes, int32_t(s->running.size()), s->opt.max_members,
                           s->opt.starve_rounds, take.data());
   if (rc) return rc;
-  // a failed add leaves that request and the ones after it waiting, so the
-  // queues stay consistent when the error is returned
+  // A request whose add fails does not hold up the others: out of device
+  // memory it waits for the next round (and fails when nothing else runs, so
+  // nothing will free memory for it); any other error fails it.
   std::deque<int32_t> keep;
   for (int32_t k = 0; k < n; k++) {
     Request& r = s->req.at(s->waiting[size_t(k)]);
     r.info.skipped = skipped[size_t(k)];
-    if (rc || !take[size_t(k)]) {
+    if (!take[size_t(k)]) {
       keep.push_back(r.id);
       continue;
     }
     int32_t mid = -1;
-    if ((rc = tr_batch_add(s->batch, &r.view, &mid))) {
+    const int arc = tr_batch_add(s->batch, &r.view, &mid);
+    if (arc == TR_ERR_NOMEM && !s->running.empty()) {
+      r.info.skipped++;
       keep.push_back(r.id);
+      continue;
+    }
+    if (arc) {
+      end_request(s, r, TR_REQ_FAILED, arc);
       continue;
     }
     r.info.state = TR_REQ_RUNNING;
@@ -142,7 +163,7 @@ int admit(tr_serve* s) {
   s->waiting.swap(keep);
   s->stats.max_members = std::max(s->stats.max_members, int32_t(s->running.size()));
   s->stats.peak_bytes = std::max(s->stats.peak_bytes, tr_batch_device_bytes(s->batch));
-  return rc;
+  return TR_OK;
 }
 
 }  // namespace
@@ -192,6 +213,8 @@ int tr_serve_create(const tr_serve_options* opt, tr_serve** out) {
 
 int tr_serve_submit(tr_serve* s, const tr_case_view* v, double arrival, int32_t* request_id) {
   if (!s || !v || !request_id) return fail(TR_ERR_INVALID, "tr_serve_submit: NULL argument");
+  // a request that could never be admitted is refused here, not at admission
+  if (tr::check_member_view(v)) return fail(TR_ERR_INVALID, std::string("tr_serve_submit: ") + tr_last_error());
   const int64_t need = tr_batch_member_bytes(v);
   if (need < 0) return fail(TR_ERR_INVALID, "tr_serve_submit: the case cannot run in the batched FP32 form");
   if (s->opt.hbm_budget_bytes > 0 && need > s->opt.hbm_budget_bytes)
@@ -273,8 +296,30 @@ int tr_serve_release(tr_serve* s, int32_t request_id) {
   if (!s) return fail(TR_ERR_INVALID, "tr_serve_release: NULL argument");
   auto it = s->req.find(request_id);
   if (it == s->req.end()) return fail(TR_ERR_INVALID, "tr_serve_release: unknown or released request");
-  if (it->second.info.state != TR_REQ_FINISHED) return fail(TR_ERR_STATE, "tr_serve_release: request not finished");
+  const int32_t st = it->second.info.state;
+  if (st == TR_REQ_WAITING || st == TR_REQ_RUNNING)
+    return fail(TR_ERR_STATE, "tr_serve_release: request still waiting or running (tr_serve_cancel ends it)");
   s->req.erase(it);
+  return TR_OK;
+}
+
+int tr_serve_cancel(tr_serve* s, int32_t request_id) {
+  if (!s) return fail(TR_ERR_INVALID, "tr_serve_cancel: NULL argument");
+  auto it = s->req.find(request_id);
+  if (it == s->req.end()) return fail(TR_ERR_INVALID, "tr_serve_cancel: unknown or released request");
+  Request& r = it->second;
+  if (r.info.state == TR_REQ_WAITING) {
+    s->waiting.erase(std::find(s->waiting.begin(), s->waiting.end(), request_id));
+  } else if (r.info.state == TR_REQ_RUNNING) {
+    const int32_t mid = r.info.member;
+    int rc = tr_batch_poll(s->batch, mid, &r.info.status);
+    if (!rc) rc = tr_batch_remove(s->batch, mid);
+    if (rc) return rc;
+    s->running.erase(mid);
+  } else {
+    return fail(TR_ERR_STATE, "tr_serve_cancel: request already ended");
+  }
+  end_request(s, r, TR_REQ_CANCELLED, TR_OK);
   return TR_OK;
 }
 

@@ -13,21 +13,35 @@ struct Face64 {
   double d2[3];  // fluctuation sent to the east (x) / north (y) cell
 };
 
+// Per-cell inputs of the face solver that the reference recomputes in every
+// face (src/kernels_flux.h:98-117 velocities, :195-200 the wet-wet clip's
+// h dx 0.5 / dt): evaluated once per cell with the same operations on the
+// same operands, so the face results keep their bits.
+struct Cell64 {
+  double u, v, hd;
+};
+__device__ __forceinline__ Cell64 cell64(double h, double qx, double qy, double dx, double dt,
+                                         double hextra) {
+  Cell64 k{0.0, 0.0, h * dx * 0.5 / dt};
+  if (h >= hextra) {
+    k.u = qx / h;
+    k.v = qy / h;
+  }
+  return k;
+}
+
 // Returns false (and leaves f untouched) when both cells are dry.  NX/NY is
 // the face normal, (1,0) for x faces and (0,1) for y faces; the products with
 // 0 and 1 are kept because they decide signed zeros exactly as the
-// reference's generic code does.
+// reference's generic code does.  c1, c2: the cells' Cell64 values.
 template <int NX, int NY>
 __device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, double qx2,
-                                             double qy1, double qy2, double z1, double z2,
-                                             double sqh1, double sqh2, double dx, double dt,
-                                             double hextra, Face64& f) {
+                                             double qy1, double qy2, const Cell64& c1,
+                                             const Cell64& c2, double z1, double z2,
+                                             double sqh1, double sqh2, Face64& f) {
   if (!(h1 > 0.0 || h2 > 0.0)) return false;
   const double nx = double(NX), ny = double(NY);
-
-  double u1 = 0.0, v1 = 0.0, u2 = 0.0, v2 = 0.0;
-  if (h1 >= hextra) { u1 = qx1 / h1; v1 = qy1 / h1; }
-  if (h2 >= hextra) { u2 = qx2 / h2; v2 = qy2 / h2; }
+  const double u1 = c1.u, v1 = c1.v, u2 = c2.u, v2 = c2.v;
 
   const double hmid = 0.5 * (h1 + h2);
   const double dh = h2 - h1;
@@ -78,8 +92,8 @@ __device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, d
 
   hp = h1 + a0;
   if (l0 * l2 < 0.0 && hp > 0.0 && h1 > 0.0 && h2 > 0.0) {
-    b0 = fmax(b0, l0 / (l0 + e2) * (a0 * l0 - h1 * dx * 0.5 / dt + e2 * h2));
-    b0 = fmin(b0, l2 / (l2 + e0) * (-a2 * l2 + h2 * dx * 0.5 / dt + e0 * h1));
+    b0 = fmax(b0, l0 / (l0 + e2) * (a0 * l0 - c1.hd + e2 * h2));
+    b0 = fmin(b0, l2 / (l2 + e0) * (-a2 * l2 + c2.hd + e0 * h1));
   }
   const double b2 = -b0;
 
@@ -122,6 +136,16 @@ __device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, d
   f.d2[1] = fabs(d21) < kEps12 ? 0.0 : d21;
   f.d2[2] = fabs(d22) < kEps12 ? 0.0 : d22;
   return true;
+}
+
+// The face with the reference's per-face argument set (the drop-in kernels).
+template <int NX, int NY>
+__device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, double qx2,
+                                             double qy1, double qy2, double z1, double z2,
+                                             double sqh1, double sqh2, double dx, double dt,
+                                             double hextra, Face64& f) {
+  return roe_face_f64<NX, NY>(h1, h2, qx1, qx2, qy1, qy2, cell64(h1, qx1, qy1, dx, dt, hextra),
+                              cell64(h2, qx2, qy2, dx, dt, hextra), z1, z2, sqh1, sqh2, f);
 }
 
 // src/kernels.h:176-185 -- per-cell CFL step (MAX_VALUE when shallower than

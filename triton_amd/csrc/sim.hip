@@ -3,7 +3,8 @@
 // TR_FORM_F64_UNFUSED replays Triton::triton<T>::simulate()
 // (src/triton_simulate.h:54-159) with the per-kernel drop-in entry points and
 // the same host-side dt / print / forcing logic, reading dt back every step as
-// the reference does.  TR_FORM_F64_FUSED and TR_FORM_F32_FUSED launch one
+// the reference does.  TR_FORM_F64_FUSED (and TR_FORM_F64_R32, the same step
+// with its state rounded to FP32) and TR_FORM_F32_FUSED launch one
 // fused kernel per step and keep dt, time and forcing indices on the device;
 // the host only looks at the control block between batches of launches.
 #include <hip/hip_runtime.h>
@@ -327,6 +328,7 @@ StepParams<double> fused_params(tr_sim* s, int64_t launch_index) {
   p.halt_on_obs = s->halt_on_obs;
   p.it_print = v.it_print;
   p.it_count0 = v.it_count;
+  p.round32 = s->form == TR_FORM_F64_R32 ? 1 : 0;
   p.bc = ExtBC{};
   if (v.num_extbc_cells > 0) {
     p.bc.n = v.num_extbc_cells;
@@ -519,7 +521,7 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
     *out = s;
     return TR_OK;
   }
-  if (form != TR_FORM_F64_UNFUSED && form != TR_FORM_F64_FUSED) {
+  if (form != TR_FORM_F64_UNFUSED && form != TR_FORM_F64_FUSED && form != TR_FORM_F64_R32) {
     set_error("tr_sim_create: unknown form");
     return TR_ERR_INVALID;
   }
@@ -564,10 +566,20 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
   }
   const int nbuf = form == TR_FORM_F64_UNFUSED ? 1 : 2;
   int rc = TR_OK;
+  // the FP32-storage yardstick starts from the state the FP32 form stores
+  std::vector<double> r32[3];
+  const double* init[3] = {v.h, v.qx, v.qy};
+  if (form == TR_FORM_F64_R32) {
+    for (int k = 0; k < 3; k++) {
+      r32[k].resize(n);
+      for (size_t i = 0; i < n; i++) r32[k][i] = double(float(init[k][i]));
+      init[k] = r32[k].data();
+    }
+  }
   for (int b = 0; b < nbuf && !rc; b++) {
-    rc = s->upload(s->h[b], v.h, n);
-    if (!rc) rc = s->upload(s->qx[b], v.qx, n);
-    if (!rc) rc = s->upload(s->qy[b], v.qy, n);
+    rc = s->upload(s->h[b], init[0], n);
+    if (!rc) rc = s->upload(s->qx[b], init[1], n);
+    if (!rc) rc = s->upload(s->qy[b], init[2], n);
   }
   if (!rc) rc = s->upload(s->dem, v.dem, n);
   if (!rc) rc = s->upload(s->nman2, v.nman2, n);
@@ -821,3 +833,4 @@ int64_t tr_sim_device_bytes(const tr_sim* s) {
 void tr_sim_destroy(tr_sim* s) { delete s; }
 
 }  // extern "C"
+

@@ -97,6 +97,7 @@ struct StepParams {
   int it_print;             // the reference's log cadence (the dt sum resets there)
   long long it_count0;      // iterations before the start (hotstart)
   ExtBC bc;                 // external boundary segments (bc.n == 0: none)
+  int round32;              // FP64 form: round updated h, qx, qy to FP32 (TR_FORM_F64_R32)
 };
 
 }  // namespace tr

@@ -48,7 +48,10 @@ __global__ __launch_bounds__(64) void k_close_f64(StepParams<double> p, int afte
   else close_one(p, p.cur, after_step != 0);
 }
 
-template <bool kOpen, bool kBC>
+// kR32: every updated h, qx, qy is rounded to FP32 before it is used or
+// stored (FP64 arithmetic on FP32-stored state: the accuracy yardstick of
+// the FP32 form, TR_FORM_F64_R32).
+template <bool kOpen, bool kBC, bool kR32>
 __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const int lane = threadIdx.x;
   const StepCtrl* cc = p.ctrl + p.cur;
@@ -77,8 +80,9 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const int e_min = by == 0 ? 0 : e_lo;
   const int e_max = by == p.nby - 1 ? R - 1 : e_hi;
 
-  // previous row of the input state
+  // previous row of the input state and its per-cell face inputs
   double ph = 0.0, pqx = 0.0, pqy = 0.0, pz = 0.0, psq = 0.0, pn2 = 0.0;
+  Cell64 pk{0.0, 0.0, 0.0};
   double prate = 0.0;
   // face contributions of the previous row: east side of the x face (rhs*0),
   // west side (rhs*1), north y face (rhs*0)
@@ -108,16 +112,18 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
       }
     }
     const double ssq = fmax(sqrt(sh), 0.0);
+    const Cell64 sk = cell64(sh, sqx, sqy, dx, dt, hextra);
 
     // B: x face (c, c+1) of row r
     double x0h = 0.0, x0x = 0.0, x0y = 0.0, x1h, x1x, x1y;
     {
       const double hR = from_right(sh), qxR = from_right(sqx), qyR = from_right(sqy);
       const double zR = from_right(sz), sqR = from_right(ssq);
+      const Cell64 kR{from_right(sk.u), from_right(sk.v), from_right(sk.hd)};
       double e0 = 0.0, e1 = 0.0, e2 = 0.0;
       if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
         Face64 f;
-        if (roe_face_f64<1, 0>(sh, hR, sqx, qxR, sqy, qyR, sz, zR, ssq, sqR, dx, dt, hextra, f)) {
+        if (roe_face_f64<1, 0>(sh, hR, sqx, qxR, sqy, qyR, sk, kR, sz, zR, ssq, sqR, f)) {
           x0h = dt * f.d1[0] / dx;
           x0x = dt * f.d1[1] / dx;
           x0y = dt * f.d1[2] / dx;
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     double y0h = 0.0, y0x = 0.0, y0y = 0.0, y1h = 0.0, y1x = 0.0, y1y = 0.0;
     if (row_ok && r >= 1 && col_int) {
       Face64 f;
-      if (roe_face_f64<0, 1>(sh, ph, sqx, pqx, sqy, pqy, sz, pz, ssq, psq, dx, dt, hextra, f)) {
+      if (roe_face_f64<0, 1>(sh, ph, sqx, pqx, sqy, pqy, sk, pk, sz, pz, ssq, psq, f)) {
         y0h = dt * f.d1[0] / dx;
         y0x = dt * f.d1[1] / dx;
         y0y = dt * f.d1[2] / dx;
@@ -168,6 +174,11 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
         if (p.src_row[k] == ru && p.src_col[k] == c) nh = nh + src_add[k];
       }
     }
+    if (kR32) {
+      nh = double(float(nh));
+      nqx = double(float(nqx));
+      nqy = double(float(nqy));
+    }
     if (kOpen) {
       // transmissive ghosts: west/east columns first, then north/south rows
       const double rh = from_right(nh), rqx = from_right(nqx), rqy = from_right(nqy);
@@ -194,6 +205,11 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
         if ((lev < zE && hE < kEps12) || (lev < zW && hW < kEps12)) qxq = 0.0;
         if ((lev < bz && bh < kEps12) || (lev < pz && nh < kEps12)) qyq = 0.0;
       }
+      if (kR32) {  // values the boundary segments wrote
+        hq = double(float(hq));
+        qxq = double(float(qxq));
+        qyq = double(float(qyq));
+      }
       if (col_mine) {
         const long long o = (long long)e * C + c;
         p.h_out[o] = hq;
@@ -209,7 +225,7 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     // rotate the window
     bh = ah; bz = az;
     ah = nh; aqx = nqx; aqy = nqy; az = pz;
-    ph = sh; pqx = sqx; pqy = sqy; pz = sz; psq = ssq; pn2 = sn2; prate = srate;
+    ph = sh; pqx = sqx; pqy = sqy; pz = sz; psq = ssq; pn2 = sn2; prate = srate; pk = sk;
     px0h = x0h; px0x = x0x; px0y = x0y; px1h = x1h; px1x = x1x; px1y = x1y;
     py0h = y0h; py0x = y0x; py0y = y0y;
   }
@@ -229,10 +245,17 @@ int launch_prime_f64(const StepParams<double>& p, hipStream_t s) {
 int launch_step_f64(const StepParams<double>& p, hipStream_t s) {
   dim3 grid(p.nblocks);
   const bool bc = p.bc.n > 0;
-  if (p.open_bc && bc) hipLaunchKernelGGL((k_step_f64<true, true>), grid, dim3(kLanes), 0, s, p);
-  else if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false>), grid, dim3(kLanes), 0, s, p);
-  else if (bc) hipLaunchKernelGGL((k_step_f64<false, true>), grid, dim3(kLanes), 0, s, p);
-  else hipLaunchKernelGGL((k_step_f64<false, false>), grid, dim3(kLanes), 0, s, p);
+  if (p.round32) {
+    if (p.open_bc && bc) hipLaunchKernelGGL((k_step_f64<true, true, true>), grid, dim3(kLanes), 0, s, p);
+    else if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false, true>), grid, dim3(kLanes), 0, s, p);
+    else if (bc) hipLaunchKernelGGL((k_step_f64<false, true, true>), grid, dim3(kLanes), 0, s, p);
+    else hipLaunchKernelGGL((k_step_f64<false, false, true>), grid, dim3(kLanes), 0, s, p);
+  } else {
+    if (p.open_bc && bc) hipLaunchKernelGGL((k_step_f64<true, true, false>), grid, dim3(kLanes), 0, s, p);
+    else if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false, false>), grid, dim3(kLanes), 0, s, p);
+    else if (bc) hipLaunchKernelGGL((k_step_f64<false, true, false>), grid, dim3(kLanes), 0, s, p);
+    else hipLaunchKernelGGL((k_step_f64<false, false, false>), grid, dim3(kLanes), 0, s, p);
+  }
   hipLaunchKernelGGL(k_close_f64, dim3(1), dim3(kLanes), 0, s, p, 1);
   return launch_check("step_f64");
 }

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import Case, SimStatus, check, lib
 
-WAITING, RUNNING, FINISHED = 0, 1, 2
+WAITING, RUNNING, FINISHED, FAILED, CANCELLED = 0, 1, 2, 3, 4
 
 
 class ServeOptions(C.Structure):
@@ -37,7 +37,7 @@ class ServeOptions(C.Structure):
 
 class ServeRequest(C.Structure):
     _fields_ = [("state", C.c_int32), ("member", C.c_int32), ("bytes", C.c_int64), ("cells", C.c_int64),
-                ("skipped", C.c_int32), ("reserved", C.c_int32), ("arrival", C.c_double),
+                ("skipped", C.c_int32), ("error", C.c_int32), ("arrival", C.c_double),
                 ("admitted", C.c_double), ("done", C.c_double), ("status", SimStatus)]
 
 
@@ -46,7 +46,7 @@ class ServeStatsC(C.Structure):
                 ("running", C.c_int64), ("cell_steps", C.c_int64), ("rounds", C.c_int64),
                 ("peak_bytes", C.c_int64), ("max_members", C.c_int32), ("reserved", C.c_int32),
                 ("busy_s", C.c_double), ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double),
-                ("max_latency_s", C.c_double)]
+                ("max_latency_s", C.c_double), ("failed", C.c_int64), ("cancelled", C.c_int64)]
 
 
 def _serve_lib() -> C.CDLL:
@@ -63,6 +63,7 @@ def _serve_lib() -> C.CDLL:
             "tr_serve_poll": (C.c_int, [vp, i32, C.POINTER(ServeRequest)]),
             "tr_serve_result": (C.c_int, [vp, i32] + [C.POINTER(C.c_double)] * 3),
             "tr_serve_release": (C.c_int, [vp, i32]),
+            "tr_serve_cancel": (C.c_int, [vp, i32]),
             "tr_serve_stats_get": (C.c_int, [vp, C.POINTER(ServeStatsC)]),
             "tr_serve_destroy": (None, [vp]),
             "tr_serve_admit": (C.c_int, [C.POINTER(i64), C.POINTER(i32), i32, i64, i64, i32, i32, i32,
@@ -156,6 +157,7 @@ class Server:
         self._views: dict[int, object] = {}   # their views stay valid until admission
         self._tags: dict[int, object] = {}
         self._done: dict[int, Request] = {}
+        self._ended: dict[int, tuple] = {}    # failed or cancelled: (state, TR_* code, tag)
         self._latencies: list[float] = []
 
     def close(self) -> None:
@@ -216,6 +218,12 @@ class Server:
             p = self.poll(r)
             if p.state == RUNNING:
                 self._views.pop(r, None)
+            if p.state in (FAILED, CANCELLED):
+                self._cases.pop(r)
+                self._views.pop(r, None)
+                self._ended[r] = (p.state, int(p.error), self._tags.pop(r, None))
+                check(L.tr_serve_release(self._h, int(r)))
+                continue
             if p.state != FINISHED:
                 continue
             case = self._cases.pop(r)
@@ -245,6 +253,15 @@ class Server:
         while self.pending():
             if not self.step() and self.waiting:
                 raise RuntimeError("waiting requests cannot be admitted")
+
+    def cancel(self, rid: int) -> None:
+        """End a waiting or running request now (tr_serve_cancel)."""
+        check(_serve_lib().tr_serve_cancel(self._h, int(rid)))
+        self._collect()
+
+    def ended(self, rid: int):
+        """(state, TR_* code, tag) of a failed or cancelled request, or None."""
+        return self._ended.get(rid)
 
     def result(self, rid: int) -> Request:
         return self._done[rid]
