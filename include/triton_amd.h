@@ -227,6 +227,40 @@ void *tr_sim_stream(tr_sim *s);
 int64_t tr_sim_device_bytes(const tr_sim *s);
 void tr_sim_destroy(tr_sim *s);
 
+/* One FP64 case on device buffers the caller owns (tr_sim_run_f64_noalloc).
+ * Arrays are device pointers, rows x cols with the ghost ring; h, qx, qy
+ * are ping-pong pairs: [0] holds the initial state and the state reached is
+ * in [steps & 1]. */
+typedef struct tr_dev_case_f64 {
+  int32_t rows, cols;
+  double dx, courant, hextra;
+  int32_t open_boundaries;
+  int32_t checkpoint_id;         /* print intervals done before sim_start */
+  double sim_start, sim_duration, print_interval, init_dt;
+  int64_t it_count;              /* iterations before the start */
+  double *h[2], *qx[2], *qy[2];
+  const double *dem, *nman2;     /* bed (walls 1e6 or copies), squared Manning n */
+  double *maxh;                  /* NULL: not tracked */
+  int32_t num_runoffs, roff_nrows, roff_row0;
+  const int32_t *runoff_id;      /* NULL: zone 0 everywhere */
+  const double *roff_time;       /* [roff_nrows] s */
+  const double *roff_rate;       /* [num_runoffs*roff_nrows] m/s, zone-major */
+} tr_dev_case_f64;
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The run tr_sim_create + tr_sim_run give in
+ * TR_FORM_F64_FUSED (same kernels, same bits), to t_stop (<= 0:
+ * sim_duration) or max_steps (< 0: unlimited), on the caller's buffers, with
+ * no allocation: the control blocks, per-step rain depths, zone areas and
+ * volume partial sums live in a static 16 KiB device array per GPU.  At most
+ * 64 runoff zones (TR_ERR_INVALID beyond); no point sources, external
+ * boundary segments or observation points.  Each call is a whole run from
+ * h[0]; blocking on `stream`.  Calls that overlap on one GPU race for its
+ * array. */
+int tr_sim_run_f64_noalloc(const tr_dev_case_f64 *c, double t_stop, int64_t max_steps,
+                           tr_sim_status *st, void *stream);
+
 /* ---- synthetic inputs ------------------------------------------------------ */
 /* No counterpart in the reference: drains a synthetic DEM (nr x nc, row-major)
  * by priority-flood depression filling with an `eps` gradient and returns the

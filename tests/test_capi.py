@@ -40,3 +40,29 @@ def test_bad_view_rejected_without_device():
     h = C.c_void_p()
     rc = ta.lib().tr_sim_create(C.byref(v), ta.FORM_F64_FUSED, C.byref(h))
     assert rc == 1 and b"rows" in ta.lib().tr_last_error()
+
+
+def test_noalloc_refusals_without_device():
+    """tr_sim_run_f64_noalloc checks its arguments before any HIP call: more
+    runoff zones than its static array holds, or missing buffers, are refused
+    with TR_ERR_INVALID."""
+    from test_zz_no_alloc import DevCase
+    f = ta.lib().tr_sim_run_f64_noalloc
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(DevCase), C.c_double, C.c_int64, C.POINTER(ta.SimStatus), C.c_void_p]
+    dc = DevCase()
+    dc.rows, dc.cols = 10, 12
+    dc.dx, dc.courant, dc.hextra, dc.print_interval, dc.init_dt, dc.sim_duration = 1.0, 0.5, 1e-3, 1.0, 0.01, 1.0
+    fake = 0x1000
+    for k in ("h", "qx", "qy"):
+        getattr(dc, k)[0] = fake
+        getattr(dc, k)[1] = fake
+    dc.dem = dc.nman2 = dc.roff_time = dc.roff_rate = fake
+    dc.num_runoffs, dc.roff_nrows = 65, 3
+    st = ta.SimStatus()
+    assert f(C.byref(dc), 0.0, -1, C.byref(st), None) == ta.TR_ERR_INVALID
+    assert b"64" in ta.lib().tr_last_error()
+    dc.num_runoffs = 1
+    dc.h[1] = None
+    assert f(C.byref(dc), 0.0, -1, C.byref(st), None) == ta.TR_ERR_INVALID
+    assert b"state buffers" in ta.lib().tr_last_error()

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -834,3 +835,253 @@ void tr_sim_destroy(tr_sim* s) { delete s; }
 
 }  // extern "C"
 
+// ---- fused FP64 run on caller buffers, with no allocation -----------------
+// tr_sim_run_f64_noalloc: the kernels of the fused FP64 form (k_prime_f64,
+// k_step_f64, k_close_f64) over device buffers the caller owns.  What the
+// form otherwise allocates per simulation -- the two control blocks with
+// their per-XCD dt minima, the rain depth each zone gains per step, the zone
+// areas and the volume partial sums -- lives in `scratch`, a file-scope
+// device array each GPU holds once for the life of the process.  Host code
+// uses the array only through the address hipGetSymbolAddress returns for
+// the current device (the array's own name is a host-side stand-in).  Every
+// call on a GPU shares that GPU's array: calls that overlap on one GPU race,
+// and each call starts by clearing what must start at zero.
+
+namespace {
+
+constexpr int kNoAllocMaxZones = 64;
+constexpr size_t kScratchBytes = 16384;
+// byte offsets into the scratch array
+constexpr size_t kOffCtrl = 0;                                      // StepCtrl[2]
+constexpr size_t kOffZoneAdd = 4096;                                // double[2][64]
+constexpr size_t kOffZoneCount = kOffZoneAdd + 2 * kNoAllocMaxZones * 8;  // u64[64]
+constexpr size_t kOffZoneArea = kOffZoneCount + kNoAllocMaxZones * 8;     // double[64]
+constexpr size_t kOffFlag = kOffZoneArea + kNoAllocMaxZones * 8;          // int
+constexpr size_t kOffVolPart = 8192;                                // double[kVolBlocks]
+constexpr size_t kOffVolSum = kOffVolPart + kVolBlocks * 8;          // double
+static_assert(2 * sizeof(StepCtrl) <= kOffZoneAdd, "control blocks overlap the zone table");
+static_assert(kOffFlag + 4 <= kOffVolPart, "zone tables overlap the volume partials");
+static_assert(kOffVolSum + 8 <= kScratchBytes, "scratch too small");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Per-device address of `scratch`, taken once per device with that device
+// current (hipGetSymbolAddress), then used for every kernel argument.
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(unsigned char** out) {
+  int dev = 0;
+  TR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("tr_sim_run_f64_noalloc: device index out of range");
+    return TR_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TR_OK;
+}
+
+// Clears the zone counters and the flag, and writes the initial control
+// block into slot 1 (the prime reduces into it and prepares slot 0).
+__global__ void k_noalloc_init(unsigned char* sc, double sim_start, int print_id, int roff_row0) {
+  const int t = int(threadIdx.x);
+  auto* cnt = reinterpret_cast<unsigned long long*>(sc + kOffZoneCount);
+  if (t < kNoAllocMaxZones) cnt[t] = 0ull;
+  if (t == 0) {
+    *reinterpret_cast<int*>(sc + kOffFlag) = 0;
+    StepCtrl* ctrl = reinterpret_cast<StepCtrl*>(sc + kOffCtrl);
+    for (int k = 0; k < 2; k++) {
+      StepCtrl& c = ctrl[k];
+      c.simtime = sim_start;
+      c.dt = 0.0;
+      c.pad2 = 0;
+      c.halt = 0;
+      c.step = 0;
+      c.print_id = print_id;
+      c.roff_row = roff_row0;
+      c.idx_low = 0;
+      c.idx_high = 0;
+      c.finished = 0;
+      c.observed = 0;
+      c.vol_in = 0.0;
+      c.step_vol_in = 0.0;
+      c.last_obs = sim_start;
+      c.dt_acc = 0.0;
+      reset_dtmin(c);
+    }
+  }
+}
+
+// Interior cells per runoff zone (exact integer counts), and a flag for a
+// zone id outside [0, nz).
+__global__ void k_noalloc_zone_count(const int* __restrict__ rid, int rows, int cols, int nz,
+                                     unsigned char* sc) {
+  auto* cnt = reinterpret_cast<unsigned long long*>(sc + kOffZoneCount);
+  const int n_int = (rows - 2) * (cols - 2);
+  for (int k = int(blockIdx.x * blockDim.x + threadIdx.x); k < n_int; k += int(gridDim.x * blockDim.x)) {
+    const int r = k / (cols - 2) + 1, c = k % (cols - 2) + 1;
+    const int z = rid ? rid[(long long)r * cols + c] : 0;
+    if (z < 0 || z >= nz) atomicOr(reinterpret_cast<int*>(sc + kOffFlag), 1);
+    else atomicAdd(&cnt[z], 1ull);
+  }
+}
+
+// zone area = cells x dx^2, as the host form computes it
+__global__ void k_noalloc_zone_area(unsigned char* sc, int nz, double dx) {
+  const int z = int(threadIdx.x);
+  if (z >= nz) return;
+  const auto* cnt = reinterpret_cast<const unsigned long long*>(sc + kOffZoneCount);
+  reinterpret_cast<double*>(sc + kOffZoneArea)[z] = double(cnt[z]) * (dx * dx);
+}
+
+}  // namespace
+
+extern "C" int tr_sim_run_f64_noalloc(const tr_dev_case_f64* dc, double t_stop, int64_t max_steps,
+                                      tr_sim_status* st, void* stream) {
+  if (!dc || !st) {
+    set_error("tr_sim_run_f64_noalloc: NULL argument");
+    return TR_ERR_INVALID;
+  }
+  const tr_dev_case_f64& c = *dc;
+  if (c.rows < 3 || c.cols < 3 || int64_t(c.rows) * c.cols > int64_t(INT32_MAX)) {
+    set_error("tr_sim_run_f64_noalloc: rows and cols must be >= 3 and the grid under 2^31 cells");
+    return TR_ERR_INVALID;
+  }
+  for (int k = 0; k < 2; k++)
+    if (!c.h[k] || !c.qx[k] || !c.qy[k]) {
+      set_error("tr_sim_run_f64_noalloc: both state buffers of h, qx and qy are needed");
+      return TR_ERR_INVALID;
+    }
+  if (!c.dem || !c.nman2) {
+    set_error("tr_sim_run_f64_noalloc: dem and nman2 are needed");
+    return TR_ERR_INVALID;
+  }
+  if (!(c.dx > 0) || !(c.courant > 0) || !(c.print_interval > 0) || !(c.init_dt > 0)) {
+    set_error("tr_sim_run_f64_noalloc: dx, courant, print_interval and init_dt must be positive");
+    return TR_ERR_INVALID;
+  }
+  if (c.num_runoffs < 0 || c.num_runoffs > kNoAllocMaxZones) {
+    set_error("tr_sim_run_f64_noalloc: " + std::to_string(c.num_runoffs) +
+              " runoff zones; at most 64 fit the static scratch array");
+    return TR_ERR_INVALID;
+  }
+  if (c.num_runoffs > 0 && (c.roff_nrows < 2 || !c.roff_time || !c.roff_rate || c.roff_row0 < 0 ||
+                            c.roff_row0 >= c.roff_nrows - 1)) {
+    set_error("tr_sim_run_f64_noalloc: runoff needs time and rate tables of at least two rows");
+    return TR_ERR_INVALID;
+  }
+  unsigned char* sc = nullptr;
+  RC(scratch_for_current_device(&sc));
+  hipStream_t s = as_stream(stream);
+  StepCtrl* ctrl = reinterpret_cast<StepCtrl*>(sc + kOffCtrl);
+  int print_id = c.checkpoint_id;
+  if (print_id == 0 && c.sim_start > 0.0) print_id = int(c.sim_start / c.print_interval);
+  hipLaunchKernelGGL(k_noalloc_init, dim3(1), dim3(64), 0, s, sc, c.sim_start, print_id,
+                     c.num_runoffs > 0 ? c.roff_row0 : 0);
+  if (c.num_runoffs > 0) {
+    const int64_t n_int = int64_t(c.rows - 2) * (c.cols - 2);
+    hipLaunchKernelGGL(k_noalloc_zone_count, dim3(unsigned(std::min<int64_t>((n_int + 255) / 256, 4096))),
+                       dim3(256), 0, s, c.runoff_id, c.rows, c.cols, c.num_runoffs, sc);
+    hipLaunchKernelGGL(k_noalloc_zone_area, dim3(1), dim3(kNoAllocMaxZones), 0, s, sc, c.num_runoffs, c.dx);
+    RC(launch_check("noalloc zones"));
+    int flag = 0;
+    TR_HIP(hipMemcpyAsync(&flag, sc + kOffFlag, sizeof(int), hipMemcpyDeviceToHost, s));
+    TR_HIP(hipStreamSynchronize(s));
+    if (flag) {
+      set_error("tr_sim_run_f64_noalloc: runoff zone id out of range");
+      return TR_ERR_INVALID;
+    }
+  }
+  const int strip = tr_default_strip_rows(c.rows, c.cols);
+  const double stop = t_stop > 0 ? std::min(t_stop, c.sim_duration) : c.sim_duration;
+  const int64_t step_stop = max_steps < 0 ? INT64_MAX : max_steps;
+  StepParams<double> p{};
+  p.rows = c.rows;
+  p.cols = c.cols;
+  p.strip_rows = strip;
+  p.nbx = fused_grid_x(c.cols);
+  p.nby = (c.rows - 2 + strip - 1) / strip;
+  p.nblocks = unsigned(p.nbx) * unsigned(p.nby);
+  p.dx = c.dx;
+  p.hextra = c.hextra;
+  p.cn_dx = c.courant * c.dx;
+  p.dem = c.dem;
+  p.nman2 = c.nman2;
+  p.maxh = c.maxh;
+  p.runoff_id = c.runoff_id;
+  p.roff_rate = c.roff_rate;
+  p.roff_time = c.roff_time;
+  p.roff_nrows = c.roff_nrows;
+  p.num_runoffs = c.num_runoffs;
+  p.zone_area = reinterpret_cast<const double*>(sc + kOffZoneArea);
+  p.zone_add = c.num_runoffs > 0 ? reinterpret_cast<double*>(sc + kOffZoneAdd) : nullptr;
+  p.nsrc = 0;
+  p.open_bc = c.open_boundaries;
+  p.ctrl = ctrl;
+  p.sim_duration = c.sim_duration;
+  p.print_interval = c.print_interval;
+  p.init_dt = c.init_dt;
+  p.t_stop = stop;
+  p.step_stop = step_stop;
+  p.it_print = 0;
+  p.it_count0 = c.it_count;
+  p.bc = ExtBC{};
+  p.round32 = 0;
+  auto set_bufs = [&](int64_t launch) {
+    const int a = int(launch & 1);
+    p.cur = a;
+    p.h_in = c.h[a];
+    p.qx_in = c.qx[a];
+    p.qy_in = c.qy[a];
+    p.h_out = c.h[a ^ 1];
+    p.qx_out = c.qx[a ^ 1];
+    p.qy_out = c.qy[a ^ 1];
+  };
+  // the prime reduces the initial dt into slot 1 and prepares slot 0
+  set_bufs(0);
+  p.cur = 1;
+  RC(launch_prime_f64(p, s));
+  StepCtrl hc;
+  int64_t launched = 0, chunk = 32;
+  for (;;) {
+    TR_HIP(hipMemcpyAsync(&hc, ctrl + (launched & 1), sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+    TR_HIP(hipStreamSynchronize(s));
+    if (hc.halt) break;
+    for (int64_t i = 0; i < chunk; i++) {
+      set_bufs(launched + i);
+      RC(launch_step_f64(p, s));
+    }
+    launched += chunk;
+    chunk = std::min<int64_t>(chunk * 2, 1024);
+  }
+  // volume of the state reached (buffer steps & 1), deterministic order
+  double* vpart = reinterpret_cast<double*>(sc + kOffVolPart);
+  double* vsum = reinterpret_cast<double*>(sc + kOffVolSum);
+  hipLaunchKernelGGL(k_volume_part, dim3(kVolBlocks), dim3(256), 0, s, c.h[hc.step & 1], c.rows, c.cols,
+                     c.dx * c.dx, vpart);
+  hipLaunchKernelGGL(k_volume_sum, dim3(1), dim3(64), 0, s, vpart, kVolBlocks, vsum);
+  RC(launch_check("noalloc volume"));
+  double vol = 0.0;
+  TR_HIP(hipMemcpyAsync(&vol, vsum, sizeof(double), hipMemcpyDeviceToHost, s));
+  TR_HIP(hipStreamSynchronize(s));
+  std::memset(st, 0, sizeof(*st));
+  st->simtime = hc.simtime;
+  st->dt = hc.dt;
+  st->steps = hc.step;
+  st->print_id = hc.print_id;
+  st->finished = hc.finished;
+  st->volume = vol;
+  st->volume_in = hc.vol_in;
+  st->last_obs = hc.last_obs;
+  st->average_dt = hc.dt_acc;
+  st->it_count = int64_t(c.it_count) + hc.step;
+  return TR_OK;
+}
