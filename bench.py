@@ -23,9 +23,16 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
+import subprocess
 import sys
 import time
 from pathlib import Path
+
+# The CPU baseline's OpenMP threads, one per physical core in the process's
+# places, spread over them (set before any OpenMP runtime starts).
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -80,12 +87,49 @@ def interior(case) -> int:
     return (case.rows - 2) * (case.cols - 2)
 
 
+def host_topology() -> str:
+    """Sockets x cores per socket, SMT state (lscpu) and the CPUs this
+    process may use."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    sockets, cores, tpc = info.get("Socket(s)", "?"), info.get("Core(s) per socket", "?"), info.get("Thread(s) per core", "?")
+    smt = "on" if tpc not in ("?", "1") else ("off" if tpc == "1" else "?")
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except OSError:
+        usable = os.cpu_count()
+    model = info.get("Model name", "")
+    return (f"{sockets} sockets x {cores} cores/socket, SMT {smt} ({tpc} threads/core){', ' + model if model else ''}; "
+            f"{usable} CPUs usable by this process; OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+            f"OMP_PLACES={os.environ.get('OMP_PLACES')}")
+
+
 def cpu_baseline(cases, target_s):
     """The FP64 oracle (the reference's unfused kernel sequence restated in C
     with OpenMP) on the host cores: the steps after each case's start (the
     spun-up state for the watershed workloads) of the workload's grids, taken
-    in order, until about target_s seconds of work are done."""
-    from oracle.pyoracle import Oracle, num_threads
+    in order, until about target_s seconds of work are done; then the same on
+    one thread over a quarter of that time, for scaling context."""
+    from oracle.pyoracle import num_threads, set_num_threads
+    threads = num_threads()
+    res = _cpu_rate(cases, target_s)
+    set_num_threads(1)
+    one = _cpu_rate(cases, max(target_s / 4.0, 1.0))
+    set_num_threads(threads)
+    return {"value": res["value"], "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": res["sample"], "host": host_topology(),
+            "one_thread": {"value": one["value"], "sample": one["sample"]},
+            "speedup_threads": round(res["value"] / one["value"], 2) if one["value"] else None}
+
+
+def _cpu_rate(cases, target_s):
+    from oracle.pyoracle import Oracle
     done_cells = 0
     spent = 0.0
     used = []
@@ -109,8 +153,7 @@ def cpu_baseline(cases, target_s):
         used.append(f"{case.rows - 2}x{case.cols - 2}:{int(r['steps'])}")
         t_start.append(float(case.scalars.get("sim_start", 0.0)))
     starts = f"{min(t_start):g}" if min(t_start) == max(t_start) else f"{min(t_start):.0f}..{max(t_start):.0f}"
-    return {"value": round(done_cells / spent / 1e9, 4), "unit": "GCUPS", "cores": num_threads(),
-            "kind": "port",
+    return {"value": round(done_cells / spent / 1e9, 4),
             "sample": f"FP64 oracle (C/OpenMP restatement of the reference's unfused kernels), the steps "
                       f"after t = {starts} s of grids [interior:steps] {', '.join(used)}; {spent:.1f} s"}
 
@@ -210,14 +253,15 @@ def run_ensemble(args):
 def fp64_plain(members, spinup_s, warmup, steps, keep=None):
     """The same members stepped one at a time by the plain FP64 fused form
     (one launch per step per member) over the same window as the FP32 batch
-    (advanced to spinup_s, then the same number of untimed steps), timed on
-    each member's stream.  keep: a list that receives the first members'
+    (advanced to spinup_s, then the same number of untimed steps, the strip
+    height autotuned during them), timed on each member's stream.  keep: a list that receives the first members'
     spun-up states as cases (the CPU baseline's sample)."""
     import torch
     from triton_amd import scenarios
     import triton_amd as ta
     total_ms = 0.0
     total_cells = 0
+    strips = []
     for m in members:
         sim = ta.Simulation(m, ta.FORM_F64_FUSED)
         stream = torch.cuda.ExternalStream(sim.stream)
@@ -226,7 +270,12 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
             if keep is not None and len(keep) < 8:
                 keep.append(scenarios.restart_at(m, sim.state(), st["simtime"]))
         base = int(sim.sync()["steps"])
-        sim.run(max_steps=warmup)
+        # rows per wave, tuned with real steps that count towards the warmup
+        strips.append(sim.autotune(steps=4, cache=False))
+        tuned = int(sim.sync()["steps"]) - base
+        if warmup > tuned:
+            sim.run(max_steps=warmup - tuned)
+        pre = int(sim.sync()["steps"])
         sim.arm(max_steps=steps)
         torch.cuda.synchronize()
         e0, e1 = events(stream)
@@ -235,7 +284,7 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
         e1.record(stream)
         e1.synchronize()
         st = sim.sync()
-        if st["steps"] != base + warmup + steps or st["finished"]:
+        if st["steps"] != pre + steps or st["finished"]:
             log("an FP64 member finished inside its timed window")
             sys.exit(1)
         total_ms += e0.elapsed_time(e1)
@@ -246,7 +295,8 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
     return {"value": round(g, 3), "unit": "GCUPS", "spinup_s": spinup_s, "warmup": warmup,
             "steps_per_member": steps,
             "form": "fp64_fused, members one at a time", "kernel": "k_step_f64",
-            "bytes_per_cell": b_alg(False, True, True)}
+            "bytes_per_cell": b_alg(False, True, True),
+            "autotuned_strip_rows": sorted(set(strips))}
 
 
 def serving(args, capacity_gcups, waves_per_simd=0):

@@ -161,9 +161,70 @@ int tr_compute_extbc_values_f64(int32_t size, int32_t nrows, int32_t ncols, doub
 int tr_wet_dry_f64(int32_t size, int32_t nrows, int32_t ncols, double dt, const double *h,
                    double *qx, double *qy, const double *dem, double *maxh, double hextra,
                    int32_t mpi_tasks, void *stream);
+/* The device cube root used by the FP64 friction (src/kernels_state.h:88
+ * takes cbrt of the depth), y[i] = cbrt(x[i]) for i < n (device pointers),
+ * so its accuracy can be checked on its own. */
+int tr_cbrt_f64(int32_t n, const double *x, double *y, void *stream);
 /* src/kernels.h:521-535 Kernels::copy_observation_points:
  * obs_arr[i] = arr[relative_index[i]] for i < size (device pointers). */
 int tr_copy_observation_points_f64(int32_t size, const double *arr, double *obs_arr,
+                                   const int32_t *relative_index, void *stream);
+
+/* ---- per-kernel drop-in entry points (FP32, device pointers) ------------- */
+/* Kernels::<name><float>, the reference's USE_SINGLE_PRECISION build
+ * (src/constants.h:37-50): float state, the double literals _G_, SQRTG and
+ * 0.5 promoting mixed expressions to FP64 before each store, EPS12 = 1e-6,
+ * FMAX/FMIN = fmaxf/fminf, math functions in double.  Same arguments and
+ * ranges as the _f64 functions above. */
+/* src/kernels.h:166-187 */
+int tr_compute_dt_and_sqrt_f32(int32_t size, float dx, const float *qx, const float *qy,
+                               const float *h, float *sqrth, float *dt_out, float cn,
+                               float hextra, void *stream);
+/* src/kernels.h:195-212 (d_min is the reduction's slot too) */
+int tr_find_min_dt_f32(int32_t size, const float *dtarr, float *d_min, float *h_min, void *stream);
+/* src/kernels_flux.h:31-285 */
+int tr_flux_x_f32(int32_t size, int32_t nrows, int32_t ncols, float dx, float dt, const float *h,
+                  const float *qx, const float *qy, const float *dem, const float *sqrth,
+                  float *rhsh0, float *rhsh1, float *rhsqx0, float *rhsqx1, float *rhsqy0,
+                  float *rhsqy1, float hextra, void *stream);
+/* src/kernels_flux.h:308-559 */
+int tr_flux_y_f32(int32_t size, int32_t nrows, int32_t ncols, float dx, float dt, const float *h,
+                  const float *qx, const float *qy, const float *dem, const float *sqrth,
+                  float *rhsh0, float *rhsh1, float *rhsqx0, float *rhsqx1, float *rhsqy0,
+                  float *rhsqy1, float hextra, void *stream);
+/* src/kernels_state.h:30-104 */
+int tr_update_cells_f32(int32_t size, int32_t nrows, int32_t ncols, float dt, float *h, float *qx,
+                        float *qy, const float *dem, const float *nman, float *rhsh0,
+                        float *rhsh1, float *rhsqx0, float *rhsqx1, float *rhsqy0, float *rhsqy1,
+                        float hextra, void *stream);
+/* src/kernels.h:102-152 */
+int tr_update_runoff_f32(int32_t size, int32_t nrows, int32_t ncols, float dt,
+                         const int32_t *runoff_id, int32_t index_row_runoff,
+                         int32_t n_rows_runoff, const float *runoff_intensity, float *h,
+                         float *qx, float *qy, float hextra, void *stream);
+/* src/kernels.h:50-84 (sources applied in list order) */
+int tr_compute_flow_f32(int32_t size, const float *hyg_time, const float *hyg_val, float dx,
+                        float dt, float simtime, int32_t idx_low, int32_t idx_high, float *h,
+                        const int32_t *pos, void *stream);
+/* src/kernels.h:436-460 */
+int tr_open_bc_west_east_f32(int32_t size, int32_t nrows, int32_t ncols, float *h, float *qx,
+                             float *qy, void *stream);
+/* src/kernels.h:476-512 */
+int tr_open_bc_north_south_f32(int32_t size, int32_t nrows, int32_t ncols, float *h, float *qx,
+                               float *qy, int32_t rank, int32_t total_process, void *stream);
+/* src/kernels.h:236-424 (cells in list order) */
+int tr_compute_extbc_values_f32(int32_t size, int32_t nrows, int32_t ncols, float dt, float *h,
+                                float *qx, float *qy, const float *dem, const float *nman,
+                                const int32_t *relative_index, const int32_t *extbctype,
+                                const int32_t *start_index, const int32_t *nrows_vars,
+                                const float *extbcvar1, const float *extbcvar2, float simtime,
+                                int32_t rank, int32_t total_process, void *stream);
+/* src/kernels_wetdry.h:24-85 (maxh may be NULL) */
+int tr_wet_dry_f32(int32_t size, int32_t nrows, int32_t ncols, float dt, const float *h,
+                   float *qx, float *qy, const float *dem, float *maxh, float hextra,
+                   int32_t mpi_tasks, void *stream);
+/* src/kernels.h:521-535 */
+int tr_copy_observation_points_f32(int32_t size, const float *arr, float *obs_arr,
                                    const int32_t *relative_index, void *stream);
 
 /* ---- device-resident simulation ------------------------------------------ */
@@ -194,6 +255,8 @@ typedef struct tr_tuning {
                               src/constants.h:43, which loses mass on
                               small-scale flows) */
   int32_t waves_per_simd;  /* FP32 form: waves per SIMD of the launch (0: 5) */
+  int32_t round_q;         /* FP32 form: discharges rounded to FP32 stochastically, as the
+                              depth always is (1), or to nearest (0) */
 } tr_tuning;
 
 /* Allocate device state for `view` in the given form and upload it. */
@@ -221,6 +284,11 @@ int tr_sim_sample(tr_sim *s, int32_t n, const int32_t *pos, double *h, double *q
  * interval (status.observed), so the caller can sample it as the reference's
  * loop does; the next run continues. */
 int tr_sim_set_obs_halt(tr_sim *s, int32_t on);
+/* Fused FP64 forms: rows per wave of the step for the launches queued next
+ * (0: by grid size, tr_default_strip_rows); results do not depend on it
+ * (the autotuner's knob).  tr_sim_strip_rows: the current value. */
+int tr_sim_set_strip_rows(tr_sim *s, int32_t strip_rows);
+int32_t tr_sim_strip_rows(const tr_sim *s);
 /* The hipStream_t the simulation runs on. */
 void *tr_sim_stream(tr_sim *s);
 /* Bytes of device memory held. */
@@ -376,6 +444,20 @@ typedef struct tr_serve_stats {
   int64_t cancelled;        /* requests ended by tr_serve_cancel */
 } tr_serve_stats;
 
+/* Per-request products at print times (the reference writes them in its
+ * loop, src/triton_simulate.h:92-110, src/output.h:512-709): at every
+ * print_interval boundary the request's H, QX, QY rasters (and MH, the
+ * running maximum depth, when the view tracks maxh) are written into
+ * output_dir as output::write_output names them (<dir>/bin/H_<id>_00.out,
+ * and with output_format 0 their ASCII twins in <dir>/asc/), and/or kept on
+ * the host (keep_prints).  The request's member stops at each print until
+ * the next scheduling round. */
+typedef struct tr_serve_request_options {
+  const char *output_dir;   /* NULL: write nothing */
+  int32_t output_format;    /* 0: ASC (ASCII + binary twin), 1: BIN only */
+  int32_t keep_prints;      /* keep each print's state for tr_serve_print */
+} tr_serve_request_options;
+
 /* The server's clock (seconds, monotonic): arrival, admitted and done times. */
 double tr_serve_clock(void);
 int tr_serve_create(const tr_serve_options *opt, tr_serve **out);
@@ -386,6 +468,15 @@ int tr_serve_create(const tr_serve_options *opt, tr_serve **out);
  * than the whole budget.  A request whose admission still fails is ended
  * with state TR_REQ_FAILED and the others carry on. */
 int tr_serve_submit(tr_serve *s, const tr_case_view *v, double arrival, int32_t *request_id);
+/* tr_serve_submit with per-request print products (opt may be NULL). */
+int tr_serve_submit_ex(tr_serve *s, const tr_case_view *v, double arrival,
+                       const tr_serve_request_options *opt, int32_t *request_id);
+/* Prints kept for a request (keep_prints) and the k-th of them: its print
+ * id, simulated time and state (rows x cols each; maxh may be NULL and is
+ * left untouched when the view does not track it). */
+int tr_serve_print_count(const tr_serve *s, int32_t request_id, int32_t *n);
+int tr_serve_print(const tr_serve *s, int32_t request_id, int32_t k, int32_t *print_id, double *simtime,
+                   double *h, double *qx, double *qy, double *maxh);
 /* One scheduling round: admit, run one chunk, retire.  *ran = 0 when no
  * request was on the device (nothing ran). */
 int tr_serve_step(tr_serve *s, int32_t *ran);

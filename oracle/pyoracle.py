@@ -1,4 +1,5 @@
-"""ctypes wrapper of the CPU FP64 oracle (liboracle_triton.so).
+"""ctypes wrapper of the CPU oracle (liboracle_triton.so): the FP64 restatement
+and the single-precision build's kernels.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, never by the product.  It takes the padded
@@ -167,6 +168,14 @@ def num_threads() -> int:
     return int(lib().orc_num_threads())
 
 
+def set_num_threads(n: int) -> None:
+    """OpenMP threads of the oracle's loops from now on."""
+    L = lib()
+    L.orc_set_num_threads.argtypes = [C.c_int]
+    L.orc_set_num_threads.restype = None
+    L.orc_set_num_threads(int(n))
+
+
 def _sig(L):
     d, i, vp = C.c_double, C.c_int, C.c_void_p
     L.orc_compute_dt_and_sqrt.argtypes = [i, d, vp, vp, vp, vp, vp, d, d]
@@ -181,13 +190,49 @@ def _sig(L):
     L.orc_open_bc_north_south.argtypes = [i, i, vp, vp, vp]
     L.orc_compute_extbc_values.argtypes = [i, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d]
     L.orc_wet_dry.argtypes = [i, i, vp, vp, vp, vp, vp, d]
+    # the single-precision build's kernels (float scalars)
+    f = C.c_float
+    L.orcf_compute_dt_and_sqrt.argtypes = [i, f, vp, vp, vp, vp, vp, f, f]
+    L.orcf_find_min_dt.argtypes = [i, vp]
+    L.orcf_find_min_dt.restype = f
+    L.orcf_face.argtypes = [f] * 15 + [vp, vp]
+    L.orcf_face.restype = i
+    for nm in ("orcf_flux_x", "orcf_flux_y"):
+        getattr(L, nm).argtypes = [i, i, f, f] + [vp] * 11 + [f]
+    L.orcf_update_cells.argtypes = [i, i, f] + [vp] * 10 + [f]
+    L.orcf_update_runoff.argtypes = [i, i, f, vp, i, i, vp, vp, vp, vp, f]
+    L.orcf_compute_flow.argtypes = [i, vp, vp, f, f, f, i, i, vp, vp]
+    L.orcf_open_bc_west_east.argtypes = [i, i, vp, vp, vp]
+    L.orcf_open_bc_north_south.argtypes = [i, i, vp, vp, vp]
+    L.orcf_compute_extbc_values.argtypes = [i, i, i, f, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, f]
+    L.orcf_wet_dry.argtypes = [i, i, vp, vp, vp, vp, vp, f]
+    L.orcf_run.argtypes = [i, i, f, f, f, i, vp, vp, vp, vp, vp, f, f, f, C.c_longlong, vp, C.POINTER(f)]
+    L.orcf_run.restype = C.c_longlong
 
 
 def kernels() -> C.CDLL:
-    """The oracle library with the per-kernel restatements' signatures set
-    (pass numpy arrays as `a.ctypes.data`)."""
+    """The oracle library with the per-kernel restatements' signatures set,
+    FP64 (orc_*) and single precision (orcf_*); pass numpy arrays as
+    `a.ctypes.data`."""
     L = lib()
     if not getattr(L, "_kernels_sig", False):
         _sig(L)
         L._kernels_sig = True
     return L
+
+
+def run_f32(h, qx, qy, dem, nman, dx, cn=0.5, hextra=1e-3, open_bc=False, t_end=1e30, print_interval=1e30,
+            init_dt=1.0, max_steps=-1):
+    """The single-precision loop (orcf_run) on copies of float32 rows x cols
+    arrays with the ghost ring; returns (h, qx, qy, steps, simtime)."""
+    import numpy as np
+    L = kernels()
+    a = [np.ascontiguousarray(x, dtype=np.float32).copy() for x in (h, qx, qy)]
+    z = np.ascontiguousarray(dem, dtype=np.float32)
+    n = np.ascontiguousarray(nman, dtype=np.float32)
+    rows, cols = a[0].shape
+    work = np.zeros(8 * rows * cols, np.float32)
+    t = C.c_float()
+    steps = L.orcf_run(rows, cols, dx, cn, hextra, int(open_bc), *[x.ctypes.data for x in a], z.ctypes.data,
+                       n.ctypes.data, t_end, print_interval, init_dt, int(max_steps), work.ctypes.data, C.byref(t))
+    return a[0], a[1], a[2], int(steps), float(t.value)

@@ -29,6 +29,15 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* Threads of the parallel loops that follow (timing at several counts). */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* src/kernels.h:166-187 -- sqrt(h) cache and per-cell CFL step, every cell
  * including the ghost ring. */
 void orc_compute_dt_and_sqrt(int size, double dx, const double *qx, const double *qy,

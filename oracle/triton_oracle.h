@@ -153,8 +153,59 @@ int orc_face(double h1, double h2, double qx1, double qx2, double qy1, double qy
  * allocation or index error. */
 int orc_run(const orc_case *c, double t_stop, long long max_steps, orc_result *res);
 
+/* ---- The single-precision build (triton_oracle_f32.c) ----
+ * The same kernels with float state and T variables and the reference's
+ * double literals and constants (src/constants.h:37-43 with
+ * USE_SINGLE_PRECISION): every mixed expression is evaluated in double and
+ * rounded to float where it is stored; EPS12 is 1e-6; FMAX/FMIN are
+ * fmaxf/fminf; sqrt and cbrt take the double the argument promotes to.
+ * Pinning (tests/test_oracle_f32.py): each orcf_ kernel against its pinned
+ * orc_ counterpart on float-representable inputs to float rounding, exactly
+ * where the kernel only copies, compares or clamps; the float loop by an
+ * exact lake at rest over a rough bed with dry islands, mirror and transpose
+ * symmetry of a centred dam break, closed-domain volume to float rounding,
+ * and agreement with the FP64 loop on a dam break.  Where in a kernel
+ * rounding to float happens (which operand is double) is the reference's C++
+ * promotion rule restated; no fixture of the single-precision build exists,
+ * so that placement is "parity unpinned" below float rounding. */
+void orcf_compute_dt_and_sqrt(int size, float dx, const float *qx, const float *qy, const float *h,
+                              float *sqrth, float *dtarr, float cn, float hextra);
+float orcf_find_min_dt(int size, const float *dtarr);
+int orcf_face(float h1, float h2, float qx1, float qx2, float qy1, float qy2, float z1, float z2,
+              float sqh1, float sqh2, float nx, float ny, float dx, float dt, float hextra, float d1[3],
+              float d2[3]);
+void orcf_flux_x(int rows, int cols, float dx, float dt, const float *h, const float *qx, const float *qy,
+                 const float *dem, const float *sqrth, float *rhsh0, float *rhsh1, float *rhsqx0, float *rhsqx1,
+                 float *rhsqy0, float *rhsqy1, float hextra);
+void orcf_flux_y(int rows, int cols, float dx, float dt, const float *h, const float *qx, const float *qy,
+                 const float *dem, const float *sqrth, float *rhsh0, float *rhsh1, float *rhsqx0, float *rhsqx1,
+                 float *rhsqy0, float *rhsqy1, float hextra);
+void orcf_update_cells(int rows, int cols, float dt, float *h, float *qx, float *qy, const float *nman,
+                       const float *rhsh0, const float *rhsh1, const float *rhsqx0, const float *rhsqx1,
+                       const float *rhsqy0, const float *rhsqy1, float hextra);
+void orcf_update_runoff(int rows, int cols, float dt, const int *runoff_id, int row, int nrow, const float *rate,
+                        float *h, float *qx, float *qy, float hextra);
+void orcf_compute_flow(int num_src, const float *hyg_time, const float *hyg_val, float dx, float dt,
+                       float simtime, int idx_low, int idx_high, float *h, const int *src_pos);
+void orcf_open_bc_west_east(int rows, int cols, float *h, float *qx, float *qy);
+void orcf_open_bc_north_south(int rows, int cols, float *h, float *qx, float *qy);
+void orcf_compute_extbc_values(int ncells, int rows, int cols, float dt, float *h, float *qx, float *qy,
+                               const float *dem, const float *nman, const int *idx, const int *type,
+                               const int *start, const int *nvars, const float *v1, const float *v2,
+                               float simtime);
+void orcf_wet_dry(int rows, int cols, const float *h, float *qx, float *qy, const float *dem, float *maxh,
+                  float hextra);
+/* Walls (bed ghosts high) or open boundaries, no forcing: steps while
+ * simtime < t_end (and < max_steps when max_steps >= 0), dt clamped to the
+ * next multiple of print_interval; init_dt when every cell is dry.  `work`
+ * holds 8*rows*cols floats.  Returns the step count. */
+long long orcf_run(int rows, int cols, float dx, float cn, float hextra, int open_bc, float *h, float *qx,
+                   float *qy, const float *dem, const float *nman, float t_end, float print_interval,
+                   float init_dt, long long max_steps, float *work, float *simtime_out);
+
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -64,4 +64,6 @@ def test_ensemble_paraboloid_job(gpu, ref_tree, tmp_path, form):
         else:
             got = np.array([[float(x) for x in ln.split()] for ln in a.decode().splitlines()])
             exp = np.array([[float(x) for x in ln.split()] for ln in ref.decode().splitlines()])
-            assert np.abs(got - exp).max() <= 1e-2
+            # depth: twice the 2.8e-3 m FP32 storage alone costs on this
+            # fixture (tests/test_gpu_f32.py measures both); discharges 1e-2
+            assert np.abs(got - exp).max() <= (5.6e-3 if var == "H" else 1e-2)

@@ -6,21 +6,25 @@ Tolerances (stated here and asserted):
   FP64 unfused and FP64 fused: the same step count and time as the oracle,
       the same wet/dry mask, and identical h, qx, qy where no cbrt is involved
       (n = 0).  With Manning friction the device cbrt and the host libm's
-      differ by an ulp on some inputs, and thin sheet flow carries that
+      differ by an ulp on some inputs (each is within an ulp of the correctly
+      rounded root, tests/test_cbrt.py), and thin sheet flow carries that
       difference along: max |dh| <= 5e-5 m, max |dq| <= 1e-4 m^2/s, mean
       |dh| <= 1e-8 m, relative mass error <= 1e-12.
-  FP32 fused (batched kernel): max |dh| <= 1e-3 m over cells wet in both
-      runs and more than 3 cells from a wet/dry edge of either run; mean |dh|
-      <= 1e-4 m over all cells; relative mass error <= 1e-6; wet/dry
-      mismatches in under 1 % of cells.  Inside the shoreline band a cell can
-      be wet in one run and dry in the other, so pointwise differences there
-      reach the depth of a thin shoreline cell; the band's max is reported
-      (and, with the max over all cells, bounded by max(1e-3, 2 x floor) where
-      the floor is how far the FP64 oracle itself moves when its inputs are
-      rounded to FP32, except on the Thacker bowl, whose moving shoreline
-      sweeps many cells: there the band max is bounded by 1e-2 m).  The
-      Thacker bowl's depth error against the analytic solution is reported
-      for every form.
+  FP32 fused (batched kernel), against the target of <= 1e-3 m max depth
+      error and <= 1e-6 relative mass error: relative mass <= 1e-6 and mean
+      |dh| <= 1e-5 m everywhere; max |dh| over every interior cell, the
+      shoreline included, <= 1e-3 m -- unless the FP32-storage yardstick
+      (TR_FORM_F64_R32: the FP64 step with h, qx, qy stored in FP32 by the FP32
+      form's own rounding rule) already departs from the oracle by more than
+      5e-4 m, in which case no FP32-storage form can be held to 1e-3 m and the
+      bound is twice the yardstick's departure; wet/dry mismatches at most
+      twice the yardstick's (plus 2).  The yardstick's figures are printed.
+  Known gap: on the 256^2 Thacker bowl the FP32 form departs by 2.5e-3 m in
+      13 shoreline cells, where the yardstick stays within 2e-6 m: FP32
+      arithmetic (not storage) cannot drain the bowl's thin shoreline films
+      the way FP64 does (a film of 6.4e-8 m that FP64 empties to 6e-20 m
+      ends at 1.0e-12 m, just above the dry threshold, and the fronts part
+      from there).  That case is a strict expected failure.
 
 Set TR_ACCURACY_REPORT=<path> to write the table as JSON lines."""
 import json
@@ -56,20 +60,23 @@ def oracle_result(name):
     return _oracle_cache[name]
 
 
-def fp32_input_floor(name):
-    """Max |dh| between the oracle on the case and on the case with h, qx,
-    qy and the bed rounded to FP32."""
+def yardstick(name):
+    """Errors of the FP32-storage yardstick form against the oracle."""
     case, ref, _ = oracle_result(name)
-    c2 = case.copy()
-    for k in ("h", "qx", "qy", "dem"):
-        c2.arrays[k] = c2.arrays[k].astype(np.float32).astype(np.float64)
-    o = Oracle.from_case(c2)
-    o.run()
-    return float(np.abs(o.state["h"][1:-1, 1:-1] - ref["h"][1:-1, 1:-1]).max())
+    with ta.Simulation(case, ta.FORM_F64_R32) as sim:
+        sim.run()
+        got = sim.state()
+    return errors(case, ref, got, case.scalars["hextra"])
 
 
-@pytest.mark.parametrize("form", list(FORMS))
-@pytest.mark.parametrize("name", list(CASES))
+THACKER_GAP = ("FP32 arithmetic cannot drain the 256^2 bowl's shoreline films as FP64 does "
+               "(2.5e-3 m in 13 cells; the FP32-storage yardstick stays within 2e-6 m)")
+PARAMS = [pytest.param(name, form, marks=pytest.mark.xfail(strict=True, reason=THACKER_GAP))
+          if (name, form) == ("thacker_256_one_period", "fp32_fused") else pytest.param(name, form)
+          for name in CASES for form in FORMS]
+
+
+@pytest.mark.parametrize("name,form", PARAMS)
 def test_error_against_oracle(gpu, name, form):
     case, ref, r = oracle_result(name)
     with ta.Simulation(case, FORMS[form]) as sim:
@@ -79,7 +86,11 @@ def test_error_against_oracle(gpu, name, form):
     e.update(case=name, form=form, steps=int(st["steps"]), oracle_steps=int(r["steps"]),
              simtime=st["simtime"], oracle_simtime=r["simtime"])
     if form == "fp32_fused":
-        e["fp32_input_floor_dh"] = fp32_input_floor(name)
+        y = yardstick(name)
+        e["yardstick_max_dh"], e["yardstick_mismatch"] = y["max_dh"], y["wet_mismatch"]
+        print(f"[accuracy] {name}: fp32 max_dh {e['max_dh']:.3g} (yardstick {y['max_dh']:.3g}), "
+              f"mismatches {e['wet_mismatch']} (yardstick {y['wet_mismatch']}), rel_mass {e['rel_mass']:.3g}",
+              flush=True)
     if name.startswith("thacker"):
         n = case.rows - 2
         _, he, _, _, _ = scenarios.thacker_exact(n=n, t=st["simtime"])
@@ -99,11 +110,8 @@ def test_error_against_oracle(gpu, name, form):
         assert e["wet_mismatch"] == 0
     else:
         assert e["simtime"] == pytest.approx(e["oracle_simtime"], rel=1e-12)
-        assert e["max_dh_inner"] <= 1e-3, e
-        assert e["mean_dh"] <= 1e-4, e
-        if name.startswith("thacker"):
-            assert e["max_dh"] <= 1e-2, e
-        else:
-            assert e["max_dh"] <= max(1e-3, 2.0 * e["fp32_input_floor_dh"]), e
-        assert e["rel_mass"] <= 1e-6, e
-        assert e["wet_mismatch"] <= 0.01 * e["cells"], e
+        assert e["rel_mass"] <= 1e-6 and e["mean_dh"] <= 1e-5, e
+        yard = e["yardstick_max_dh"]
+        bound = 1e-3 if yard <= 5e-4 else 2.0 * yard
+        assert e["max_dh"] <= bound, e
+        assert e["wet_mismatch"] <= 2 * e["yardstick_mismatch"] + 2, e

@@ -12,19 +12,20 @@ own.
 
 Watershed, 4096 x 4096 cells at 10 m (fBm bed with carved channels, five
 Manning classes, rain peaking at 50 mm/h at 1 h, four inflows, open edges),
-and the first 8 members of the 64-member ensemble (sides 1819, 741, 1949,
-303, 905, 560, 1357, 368; the ensemble's seeds, Manning scales and rain
-peaks): a 6 h or 2 h run is ~10^11-10^12 oracle cell-updates, so these are
-checked over a window of the running event.  The FP64 fused form advances
+and all 64 members of the ensemble (sides 279..1982, the ensemble's seeds,
+Manning scales and rain peaks): a 6 h or 2 h run is ~10^11-10^12 oracle
+cell-updates, so these are checked over a window of the running event.  The FP64 fused form advances
 each case to peak rain (t = 3600 s); from that state the oracle and every
 form take the same 30 steps.  FP64 fused equals FP64 unfused bit for bit
 (both use the device cbrt); against the oracle (glibc cbrt) max |dh| <=
 1e-8 m, max |dq| <= 1e-8 m^2/s, relative mass <= 1e-13 and simulated time
 to 1e-14 relative (a cbrt ulp can reach the step's smallest dt).  FP32
-(state rounded to FP32 at upload): max |dh| <= 1e-3 m away from the
-shoreline band, mean |dh| <= 1e-5 m, relative mass <= 1e-6, wet/dry
-mismatches in under 0.1 % of cells, the oracle's simulated time to 1e-8
-relative.  The
+(state rounded to FP32 at upload): max |dh| <= 1e-3 m over every interior
+cell, the shoreline's included, mean |dh| <= 1e-5 m, relative mass <= 1e-6,
+wet/dry mismatches (reported per case) in under 0.01 % of cells, the
+window's simulated time advance to 1e-5 relative (dt follows the fastest
+cell, whose FP32 state differs by round-off: up to 4e-5 relative in one
+step's dt).  The
 ensemble members run in one batch under the launch the benchmark tunes to
 (20 waves per SIMD), with hipGraph replays of 10 steps, and equal, bit for bit,
 each member run alone with the default eager launch.
@@ -75,10 +76,12 @@ def _check_f64_window(case, ref, r, got, st):
 def _check_f32_window(case, ref, r, got, st):
     e = errors(case, ref, got, case.scalars["hextra"])
     assert st["steps"] == r["steps"], (st, r)
-    assert st["simtime"] == pytest.approx(r["simtime"], rel=1e-8), (st, r)
-    assert e["max_dh_inner"] <= 1e-3 and e["mean_dh"] <= 1e-5, e
+    # dt follows the fastest cell, whose FP32 state differs by round-off
+    t0 = case.scalars["sim_start"]
+    assert st["simtime"] - t0 == pytest.approx(r["simtime"] - t0, rel=1e-5), (st, r)
+    assert e["max_dh"] <= 1e-3 and e["mean_dh"] <= 1e-5, e
     assert e["rel_mass"] <= 1e-6, e
-    assert e["wet_mismatch"] <= 1e-3 * e["cells"], e
+    assert e["wet_mismatch"] <= 1e-4 * e["cells"], e
     return e
 
 
@@ -133,10 +136,10 @@ def test_watershed_4096_window(gpu):
 
 
 def test_ensemble_members_window(gpu):
-    sizes = scenarios.ensemble_sizes(64)[:8]
-    assert sizes == [1819, 741, 1949, 303, 905, 560, 1357, 368]
-    print(f"[fullsize] ensemble members {sizes}: generate and spin up to t = 3600 s", flush=True)
-    cases = [scenarios.spun_up(c, SPINUP_S) for c in scenarios.ensemble(count=8, sizes=sizes)]
+    sizes = scenarios.ensemble_sizes(64)
+    assert sizes[:8] == [1819, 741, 1949, 303, 905, 560, 1357, 368] and len(sizes) == 64
+    print(f"[fullsize] 64 ensemble members: generate and spin up to t = {SPINUP_S:.0f} s", flush=True)
+    cases = [scenarios.spun_up(c, SPINUP_S) for c in scenarios.ensemble(count=64, sizes=sizes)]
     with ta.Batch() as b:
         b.set_launch(20, 10)
         ids = [b.add(c) for c in cases]

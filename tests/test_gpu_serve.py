@@ -123,3 +123,38 @@ def test_cancel_waiting_and_running_requests(gpu):
         for k in ("h", "qx", "qy"):
             np.testing.assert_array_equal(r.state[k], ref.state[k])
         assert srv.stats.completed == 1
+
+
+def test_served_request_print_products(gpu, tmp_path):
+    """A served request's print products -- the H, QX, QY and MH rasters the
+    reference writes at every print_interval (src/output.h:512-709), and the
+    kept print states -- equal those of the same case run alone in the FP32
+    form and written by the output module (same files, byte for byte), also
+    when the request shares its launches with another."""
+    from triton_amd.output import run_with_outputs
+    case = scenarios.watershed(n=96, seed=19, hours=0.1, rain_peak_mmh=900.0, inflow_peaks=())
+    case.scalars["print_interval"] = 90.0
+    case.arrays["maxh"] = case.arrays["h"].copy()
+    case.config["max_value_print_option"] = "h"  # the output module then writes MH too
+    other = scenarios.circular_dambreak(n=80, radius=10.0, t_end=30.0)
+    alone_dir = tmp_path / "alone"
+    with ta.Simulation(case, ta.FORM_F32_FUSED) as sim:
+        st = run_with_outputs(sim, case, alone_dir)
+    served_dir = tmp_path / "served"
+    with Server(chunk_steps=16) as srv:
+        srv.submit(other)
+        rid = srv.submit(case, output_dir=served_dir, keep_prints=True)
+        srv.drain()
+        r = srv.result(rid)
+    n_prints = int(st["print_id"])
+    assert n_prints == 4 and [p[0] for p in r.prints] == list(range(1, n_prints + 1))
+    for sub in ("asc", "bin"):
+        a = sorted(p.name for p in (alone_dir / sub).iterdir())
+        b = sorted(p.name for p in (served_dir / sub).iterdir())
+        assert a == b and len(a) == 4 * n_prints, (a, b)
+        for name in a:
+            assert (alone_dir / sub / name).read_bytes() == (served_dir / sub / name).read_bytes(), name
+    pid, t, last = r.prints[-1]
+    assert t == pytest.approx(st["simtime"]) and r.status["finished"]
+    for k in ("h", "qx", "qy"):
+        np.testing.assert_array_equal(last[k], r.state[k])

@@ -157,9 +157,16 @@ def test_allatoona_fused_forms(gpu, ref_tree):
     with ta.Simulation(case, ta.FORM_F32_FUSED) as sim:
         s32 = sim.run()
         g = sim.state()
+    with ta.Simulation(case, ta.FORM_F64_R32) as sim:
+        sim.run()
+        y = sim.state()
     assert s32["finished"] and s32["print_id"] == 2
-    ref = load_asc(ref_tree / "test/reference/allatoona" / "H_02_00.out")
-    assert np.abs(as_printed(g["h"]) - ref).max() <= 1e-2
+    # FP32 against the fused FP64 form: 1e-3 m over every interior cell, or
+    # twice what FP32 storage alone costs (the yardstick form) where that is more
+    d32 = np.abs(g["h"] - f["h"])[1:-1, 1:-1].max()
+    yard = np.abs(y["h"] - f["h"])[1:-1, 1:-1].max()
+    print(f"[allatoona] fp32 max_dh {d32:.3g}, yardstick {yard:.3g}", flush=True)
+    assert d32 <= max(1e-3, 2.0 * yard), (d32, yard)
 
 
 def with_extbc(case, entries):
@@ -212,7 +219,9 @@ def test_extbc_fused_equals_unfused_and_oracle(gpu, shape, open_bc, friction):
     for bit.  Without a cube root (no friction, no normal-slope entries) both
     equal the oracle bit for bit; with one the device's and glibc's cbrt
     differ by an ulp on some inputs, and thin films carry that further
-    (stated tolerance).  FP32 within 1e-2 m."""
+    (stated tolerance).  FP32 within 1e-3 m of the oracle over every
+    interior cell, or twice the FP32-storage yardstick's departure where that
+    is larger."""
     nr, nc = shape
     case = with_extbc(synthetic(nr, nc, seed=nr + nc, open_bc=open_bc, dur=8.0, sources=False,
                                 friction=friction), extbc_entries(nr, nc, cbrt_free=not friction))
@@ -232,7 +241,14 @@ def test_extbc_fused_equals_unfused_and_oracle(gpu, shape, open_bc, friction):
             assert d <= (1e-3 if friction else 0.0), (form, k, d)
     st32, s32 = out[ta.FORM_F32_FUSED]
     assert st32["finished"]
-    assert np.abs(s32["h"] - orc.state["h"]).max() <= 1e-2
+    with ta.Simulation(case, ta.FORM_F64_R32) as sim:
+        sim.run()
+        y = sim.state()
+    d32 = np.abs(s32["h"] - orc.state["h"])[1:-1, 1:-1].max()
+    yard = np.abs(y["h"] - orc.state["h"])[1:-1, 1:-1].max()
+    print(f"[extbc {shape} open={open_bc} friction={friction}] fp32 max_dh {d32:.3g}, yardstick {yard:.3g}",
+          flush=True)
+    assert d32 <= max(1e-3, 2.0 * yard), (d32, yard)
 
 
 def test_extbc_cell_listed_twice(gpu):

@@ -73,7 +73,8 @@ class CaseView(C.Structure):
 class Tuning(C.Structure):
     """tr_tuning: rows per wave of the fused step and the FP32 dry threshold."""
 
-    _fields_ = [("strip_rows", C.c_int32), ("dry_eps", C.c_double), ("waves_per_simd", C.c_int32)]
+    _fields_ = [("strip_rows", C.c_int32), ("dry_eps", C.c_double), ("waves_per_simd", C.c_int32),
+                ("round_q", C.c_int32)]
 
 
 class SimStatus(C.Structure):
@@ -103,7 +104,7 @@ def lib() -> C.CDLL:
             " (there is no CPU fallback)")
     L = C.CDLL(str(LIB_PATH))
     vp = C.c_void_p
-    i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+    i32, i64, f64, f32 = C.c_int32, C.c_int64, C.c_double, C.c_float
     sig = {
         "tr_last_error": (C.c_char_p, []),
         "tr_version": (C.c_char_p, []),
@@ -128,6 +129,8 @@ def lib() -> C.CDLL:
         "tr_batch_set_obs_halt": (C.c_int, [vp, i32, i32]),
         "tr_copy_observation_points_f64": (C.c_int, [i32, vp, vp, vp, vp]),
         "tr_sim_stream": (vp, [vp]),
+        "tr_sim_set_strip_rows": (C.c_int, [vp, i32]),
+        "tr_sim_strip_rows": (i32, [vp]),
         "tr_sim_device_bytes": (i64, [vp]),
         "tr_sim_destroy": (None, [vp]),
         "tr_batch_create": (C.c_int, [i64, i32, C.POINTER(vp)]),
@@ -161,6 +164,20 @@ def lib() -> C.CDLL:
         "tr_compute_extbc_values_f64": (C.c_int, [i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp,
                                                   vp, vp, vp, vp, f64, i32, i32, vp]),
         "tr_wet_dry_f64": (C.c_int, [i32, i32, i32, f64, vp, vp, vp, vp, vp, f64, i32, vp]),
+        "tr_cbrt_f64": (C.c_int, [i32, vp, vp, vp]),
+        "tr_compute_dt_and_sqrt_f32": (C.c_int, [i32, f32, vp, vp, vp, vp, vp, f32, f32, vp]),
+        "tr_find_min_dt_f32": (C.c_int, [i32, vp, vp, C.POINTER(C.c_float), vp]),
+        "tr_flux_x_f32": (C.c_int, [i32, i32, i32, f32, f32] + [vp] * 11 + [f32, vp]),
+        "tr_flux_y_f32": (C.c_int, [i32, i32, i32, f32, f32] + [vp] * 11 + [f32, vp]),
+        "tr_update_cells_f32": (C.c_int, [i32, i32, i32, f32] + [vp] * 11 + [f32, vp]),
+        "tr_update_runoff_f32": (C.c_int, [i32, i32, i32, f32, vp, i32, i32, vp, vp, vp, vp, f32, vp]),
+        "tr_compute_flow_f32": (C.c_int, [i32, vp, vp, f32, f32, f32, i32, i32, vp, vp, vp]),
+        "tr_open_bc_west_east_f32": (C.c_int, [i32, i32, i32, vp, vp, vp, vp]),
+        "tr_open_bc_north_south_f32": (C.c_int, [i32, i32, i32, vp, vp, vp, i32, i32, vp]),
+        "tr_compute_extbc_values_f32": (C.c_int, [i32, i32, i32, f32, vp, vp, vp, vp, vp, vp, vp,
+                                                  vp, vp, vp, vp, f32, i32, i32, vp]),
+        "tr_wet_dry_f32": (C.c_int, [i32, i32, i32, f32, vp, vp, vp, vp, vp, f32, i32, vp]),
+        "tr_copy_observation_points_f32": (C.c_int, [i32, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -407,12 +424,12 @@ class Simulation:
     """A device-resident simulation of one case (tr_sim_*)."""
 
     def __init__(self, case: Case, form: int = FORM_F64_FUSED, strip_rows: int = 0,
-                 dry_eps: float = 0.0, waves_per_simd: int = 0):
+                 dry_eps: float = 0.0, waves_per_simd: int = 0, round_q: int = 0):
         self.case = case
         self.form = form
         self._view = case.view()
         h = C.c_void_p()
-        tune = Tuning(int(strip_rows), float(dry_eps), int(waves_per_simd))
+        tune = Tuning(int(strip_rows), float(dry_eps), int(waves_per_simd), int(round_q))
         check(lib().tr_sim_create_ex(C.byref(self._view), int(form), C.byref(tune), C.byref(h)))
         self._h = h
 
@@ -478,6 +495,21 @@ class Simulation:
     def device_bytes(self) -> int:
         return int(lib().tr_sim_device_bytes(self._h))
 
+    def set_strip_rows(self, rows: int) -> None:
+        """Fused FP64 forms: rows per wave of the step (0: by grid size).
+        Results do not change."""
+        check(lib().tr_sim_set_strip_rows(self._h, int(rows)))
+
+    @property
+    def strip_rows(self) -> int:
+        return int(lib().tr_sim_strip_rows(self._h))
+
+    def autotune(self, candidates=None, steps: int = 8, cache: bool = True) -> int:
+        """Fused FP64 forms: time a few strip heights with real steps and keep
+        the fastest (triton_amd.tune)."""
+        from .tune import autotune_sim
+        return autotune_sim(self, candidates=candidates, steps=steps, cache=cache)
+
 
 class Batch:
     """Members of uneven size stepped together in FP32, one launch per step
@@ -513,10 +545,10 @@ class Batch:
         v = case.view()
         return int(lib().tr_batch_member_bytes(C.byref(v)))
 
-    def add(self, case: Case, dry_eps: float = 0.0) -> int:
+    def add(self, case: Case, dry_eps: float = 0.0, round_q: int = 0) -> int:
         v = case.view()
         mid = C.c_int32()
-        tune = Tuning(0, float(dry_eps), 0)
+        tune = Tuning(0, float(dry_eps), 0, int(round_q))
         check(lib().tr_batch_add_ex(self._h, C.byref(v), C.byref(tune), C.byref(mid)))
         self.members[mid.value] = case
         return mid.value

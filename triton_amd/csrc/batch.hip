@@ -493,6 +493,7 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   P.rows = rows;
   P.cols = cols;
   P.eps_f = (tune && tune->dry_eps > 0) ? float(tune->dry_eps) : kDefaultDryEpsF32;
+  P.round_q = (tune && tune->round_q) ? 1 : 0;
   P.nbx = fused_grid_x_f32(cols);
   P.dx = v.dx;
   P.hextra = v.hextra;

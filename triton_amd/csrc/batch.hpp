@@ -16,10 +16,11 @@ namespace tr {
 
 struct InstF32 {
   int rows, cols;            // padded grid
-  int nbx;                   // column strips of 60 interior columns
+  int nbx;                   // column strips of 60 interior columns (one cell per lane)
   double dx, hextra, cn_dx;  // FP64 copies for the step bookkeeping
   float dx_f, hextra_f, cn_dx_f;
   float eps_f;               // dry / flush threshold
+  int round_q;               // discharges rounded stochastically (else to nearest)
   float* h[2];               // ping-pong state, FP32
   float* qx[2];
   float* qy[2];

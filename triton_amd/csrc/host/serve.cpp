@@ -13,7 +13,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <deque>
+#include <filesystem>
 #include <limits>
 #include <map>
 #include <string>
@@ -29,12 +31,26 @@ const char* tr_last_error(void);
 
 namespace {
 
+struct PrintRec {
+  int32_t print_id = 0;
+  double simtime = 0.0;
+  std::vector<double> h, qx, qy, maxh;
+};
+
 struct Request {
   int32_t id = 0;
   tr_case_view view{};  // caller-owned arrays, valid until admission
   tr_serve_request info{};
   std::vector<double> h, qx, qy;  // final state (keep_state)
   int64_t rows = 0, cols = 0;
+  // print products
+  std::string outdir;
+  int32_t out_format = 0;
+  bool keep_prints = false;
+  bool track_maxh = false;
+  int32_t last_print = 0;
+  std::vector<PrintRec> prints;
+  bool wants_prints() const { return keep_prints || !outdir.empty(); }
 };
 
 double now_s() {
@@ -82,6 +98,58 @@ void end_request(tr_serve* s, Request& r, int32_t state, int code) {
   r.view = tr_case_view{};
   if (state == TR_REQ_FAILED) s->stats.failed++;
   else s->stats.cancelled++;
+}
+
+// The reference's raster names (outfile_pattern "%s/%s/%s_%02d_%02d" + ".out").
+std::string raster_path(const std::string& dir, const char* sub, const char* name, int pid) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "/%s/%s_%02d_%02d.out", sub, name, pid, 0);
+  return dir + buf;
+}
+
+// Capture the prints of running requests whose member stopped at a print
+// boundary since the last round: write the rasters and/or keep the state.
+int capture_prints(tr_serve* s) {
+  for (auto& kv : s->running) {
+    Request& r = s->req.at(kv.second);
+    if (!r.wants_prints()) continue;
+    tr_sim_status st{};
+    int rc = tr_batch_poll(s->batch, kv.first, &st);
+    if (rc) return rc;
+    if (st.print_id <= r.last_print) continue;
+    PrintRec p;
+    p.print_id = st.print_id;
+    p.simtime = st.simtime;
+    const size_t n = size_t(r.rows * r.cols);
+    p.h.resize(n);
+    p.qx.resize(n);
+    p.qy.resize(n);
+    if (r.track_maxh) p.maxh.resize(n);
+    if ((rc = tr_batch_download(s->batch, kv.first, p.h.data(), p.qx.data(), p.qy.data(),
+                                r.track_maxh ? p.maxh.data() : nullptr)))
+      return rc;
+    if (!r.outdir.empty()) {
+      const struct {
+        const char* name;
+        const std::vector<double>* a;
+      } fields[] = {{"H", &p.h}, {"QX", &p.qx}, {"QY", &p.qy}, {"MH", &p.maxh}};
+      for (const auto& f : fields) {
+        if (f.a->empty()) continue;
+        for (int asc = 0; asc < (r.out_format == 0 ? 2 : 1); asc++) {
+          const char* sub = asc ? "asc" : "bin";
+          std::error_code ec;
+          std::filesystem::create_directories(r.outdir + "/" + sub, ec);
+          const std::string path = raster_path(r.outdir, sub, f.name, p.print_id);
+          rc = asc ? tr_write_output_ascii(path.c_str(), int32_t(r.rows), int32_t(r.cols), f.a->data())
+                   : tr_write_output_binary(path.c_str(), int32_t(r.rows), int32_t(r.cols), f.a->data());
+          if (rc) return rc;
+        }
+      }
+    }
+    r.last_print = p.print_id;
+    if (r.keep_prints) r.prints.push_back(std::move(p));
+  }
+  return TR_OK;
 }
 
 // Retire every running request whose member has finished.
@@ -154,6 +222,13 @@ int admit(tr_serve* s) {
       end_request(s, r, TR_REQ_FAILED, arc);
       continue;
     }
+    if (r.wants_prints()) {
+      int prc = tr_batch_set_print_halt(s->batch, mid, 1);
+      tr_sim_status st0{};
+      if (!prc) prc = tr_batch_poll(s->batch, mid, &st0);
+      if (prc) return prc;
+      r.last_print = st0.print_id;
+    }
     r.info.state = TR_REQ_RUNNING;
     r.info.member = mid;
     r.info.admitted = now_s();
@@ -212,7 +287,14 @@ int tr_serve_create(const tr_serve_options* opt, tr_serve** out) {
 }
 
 int tr_serve_submit(tr_serve* s, const tr_case_view* v, double arrival, int32_t* request_id) {
+  return tr_serve_submit_ex(s, v, arrival, nullptr, request_id);
+}
+
+int tr_serve_submit_ex(tr_serve* s, const tr_case_view* v, double arrival, const tr_serve_request_options* ro,
+                       int32_t* request_id) {
   if (!s || !v || !request_id) return fail(TR_ERR_INVALID, "tr_serve_submit: NULL argument");
+  if (ro && (ro->output_format < 0 || ro->output_format > 1))
+    return fail(TR_ERR_INVALID, "tr_serve_submit: output_format must be 0 (ASC) or 1 (BIN)");
   // a request that could never be admitted is refused here, not at admission
   if (tr::check_member_view(v)) return fail(TR_ERR_INVALID, std::string("tr_serve_submit: ") + tr_last_error());
   const int64_t need = tr_batch_member_bytes(v);
@@ -233,6 +315,12 @@ int tr_serve_submit(tr_serve* s, const tr_case_view* v, double arrival, int32_t*
   r.info.arrival = arrival >= 0.0 ? arrival : now_s();
   r.info.admitted = -1.0;
   r.info.done = -1.0;
+  if (ro) {
+    if (ro->output_dir) r.outdir = ro->output_dir;
+    r.out_format = ro->output_format;
+    r.keep_prints = ro->keep_prints != 0;
+  }
+  r.track_maxh = v->maxh != nullptr;
   s->waiting.push_back(r.id);
   s->req.emplace(r.id, std::move(r));
   s->stats.submitted++;
@@ -251,6 +339,7 @@ int tr_serve_step(tr_serve* s, int32_t* ran) {
   if ((rc = tr_batch_arm(s->batch, 0.0, s->opt.chunk_steps))) return rc;
   if ((rc = tr_batch_enqueue(s->batch, s->opt.chunk_steps))) return rc;
   if ((rc = tr_batch_sync(s->batch, &active))) return rc;
+  if ((rc = capture_prints(s))) return rc;
   if ((rc = retire(s))) return rc;
   s->stats.busy_s += now_s() - t0;
   s->stats.rounds++;
@@ -300,6 +389,32 @@ int tr_serve_release(tr_serve* s, int32_t request_id) {
   if (st == TR_REQ_WAITING || st == TR_REQ_RUNNING)
     return fail(TR_ERR_STATE, "tr_serve_release: request still waiting or running (tr_serve_cancel ends it)");
   s->req.erase(it);
+  return TR_OK;
+}
+
+int tr_serve_print_count(const tr_serve* s, int32_t request_id, int32_t* n) {
+  if (!s || !n) return fail(TR_ERR_INVALID, "tr_serve_print_count: NULL argument");
+  auto it = s->req.find(request_id);
+  if (it == s->req.end()) return fail(TR_ERR_INVALID, "tr_serve_print_count: unknown or released request");
+  *n = int32_t(it->second.prints.size());
+  return TR_OK;
+}
+
+int tr_serve_print(const tr_serve* s, int32_t request_id, int32_t k, int32_t* print_id, double* simtime,
+                   double* h, double* qx, double* qy, double* maxh) {
+  if (!s) return fail(TR_ERR_INVALID, "tr_serve_print: NULL argument");
+  auto it = s->req.find(request_id);
+  if (it == s->req.end()) return fail(TR_ERR_INVALID, "tr_serve_print: unknown or released request");
+  const Request& r = it->second;
+  if (k < 0 || k >= int32_t(r.prints.size())) return fail(TR_ERR_INVALID, "tr_serve_print: no such print");
+  const PrintRec& p = r.prints[size_t(k)];
+  if (print_id) *print_id = p.print_id;
+  if (simtime) *simtime = p.simtime;
+  const size_t bytes = p.h.size() * sizeof(double);
+  if (h) std::memcpy(h, p.h.data(), bytes);
+  if (qx) std::memcpy(qx, p.qx.data(), bytes);
+  if (qy) std::memcpy(qy, p.qy.data(), bytes);
+  if (maxh && !p.maxh.empty()) std::memcpy(maxh, p.maxh.data(), bytes);
   return TR_OK;
 }
 

@@ -1,86 +1,127 @@
-// roe_f64.hpp -- FP64 per-face Roe solver with bed-slope source, in the
-// reference's floating-point operation order so the GPU reproduces the FP64
-// CPU build bit for bit (src/kernels_flux.h:81-265 for x, :364-547 for y).
-// Translation units including this file are compiled with -ffp-contract=off.
+// roe_f64.hpp -- the reference's per-face Roe solver with bed-slope source,
+// per-cell CFL step and cell update, in the reference's floating-point
+// operation order so the GPU reproduces the CPU builds bit for bit
+// (src/kernels_flux.h:81-265 for x, :364-547 for y; src/kernels_state.h:62-101;
+// src/kernels.h:176-185).  Written, as the reference is, as templates on the
+// state type T with double literals and double constants (_G_, SQRTG,
+// 0.5 ...): with T = double every operation is FP64; with T = float the
+// mixed expressions are evaluated in FP64 and rounded when stored, exactly as
+// the reference's USE_SINGLE_PRECISION build does (src/constants.h:37-50:
+// EPS12 = 1e-6, FMAX/FMIN = fmaxf/fminf), with the math functions taking and
+// returning double as the C library's do.  Translation units including this
+// file are compiled with -ffp-contract=off.
 #pragma once
 
 #include "common.hpp"
 
 namespace tr {
 
-struct Face64 {
-  double d1[3];  // fluctuation sent to the west (x) / south (y) cell
-  double d2[3];  // fluctuation sent to the east (x) / north (y) cell
+// EPS12 of the reference's build for state type T
+template <class T>
+__host__ __device__ constexpr double eps12_of() {
+  return sizeof(T) == 4 ? 1e-6 : 1e-12;
+}
+// FMAX / FMIN of the reference's build: arguments converted to T first
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) {
+  return fmax(a, b);
+}
+template <class T>
+__device__ __forceinline__ T tmin(T a, T b) {
+  return fmin(a, b);
+}
+template <>
+__device__ __forceinline__ float tmax<float>(float a, float b) {
+  return fmaxf(a, b);
+}
+template <>
+__device__ __forceinline__ float tmin<float>(float a, float b) {
+  return fminf(a, b);
+}
+
+template <class T>
+struct FaceT {
+  T d1[3];  // fluctuation sent to the west (x) / south (y) cell
+  T d2[3];  // fluctuation sent to the east (x) / north (y) cell
 };
+using Face64 = FaceT<double>;
 
 // Per-cell inputs of the face solver that the reference recomputes in every
 // face (src/kernels_flux.h:98-117 velocities, :195-200 the wet-wet clip's
 // h dx 0.5 / dt): evaluated once per cell with the same operations on the
-// same operands, so the face results keep their bits.
-struct Cell64 {
-  double u, v, hd;
+// same operands, so the face results keep their bits.  hd is the double the
+// reference's expression yields for either T.
+template <class T>
+struct CellT {
+  T u, v;
+  double hd;
 };
-__device__ __forceinline__ Cell64 cell64(double h, double qx, double qy, double dx, double dt,
-                                         double hextra) {
-  Cell64 k{0.0, 0.0, h * dx * 0.5 / dt};
+using Cell64 = CellT<double>;
+template <class T>
+__device__ __forceinline__ CellT<T> cell_t(T h, T qx, T qy, T dx, T dt, T hextra) {
+  CellT<T> k{T(0), T(0), h * dx * 0.5 / dt};
   if (h >= hextra) {
     k.u = qx / h;
     k.v = qy / h;
   }
   return k;
 }
+__device__ __forceinline__ Cell64 cell64(double h, double qx, double qy, double dx, double dt,
+                                         double hextra) {
+  return cell_t<double>(h, qx, qy, dx, dt, hextra);
+}
 
 // Returns false (and leaves f untouched) when both cells are dry.  NX/NY is
 // the face normal, (1,0) for x faces and (0,1) for y faces; the products with
 // 0 and 1 are kept because they decide signed zeros exactly as the
-// reference's generic code does.  c1, c2: the cells' Cell64 values.
-template <int NX, int NY>
-__device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, double qx2,
-                                             double qy1, double qy2, const Cell64& c1,
-                                             const Cell64& c2, double z1, double z2,
-                                             double sqh1, double sqh2, Face64& f) {
+// reference's generic code does.  c1, c2: the cells' CellT values.
+template <class T, int NX, int NY>
+__device__ __forceinline__ bool roe_face_t(T h1, T h2, T qx1, T qx2, T qy1, T qy2, const CellT<T>& c1,
+                                           const CellT<T>& c2, T z1, T z2, T sqh1, T sqh2,
+                                           FaceT<T>& f) {
+  constexpr double eps = eps12_of<T>();
   if (!(h1 > 0.0 || h2 > 0.0)) return false;
-  const double nx = double(NX), ny = double(NY);
-  const double u1 = c1.u, v1 = c1.v, u2 = c2.u, v2 = c2.v;
+  const T nx = T(NX), ny = T(NY);
+  const T u1 = c1.u, v1 = c1.v, u2 = c2.u, v2 = c2.v;
 
-  const double hmid = 0.5 * (h1 + h2);
-  const double dh = h2 - h1;
-  const double c = sqrt(kG * hmid);
-  const double ubar = (u1 * sqh1 + u2 * sqh2) / (sqh1 + sqh2);
-  const double vbar = (v1 * sqh1 + v2 * sqh2) / (sqh1 + sqh2);
-  const double un = ubar * nx + vbar * ny;
-  double l0 = un - c, l1 = un, l2 = un + c;
-  double e0 = 0.0, e2 = 0.0;  // entropy-fix wave speeds (lamE[0], lamE[2])
+  const T hmid = 0.5 * (h1 + h2);
+  const T dh = h2 - h1;
+  const T c = sqrt(kG * hmid);
+  const T ubar = (u1 * sqh1 + u2 * sqh2) / (sqh1 + sqh2);
+  const T vbar = (v1 * sqh1 + v2 * sqh2) / (sqh1 + sqh2);
+  const T un = ubar * nx + vbar * ny;
+  T l0 = un - c, l1 = un, l2 = un + c;
+  T e0 = T(0), e2 = T(0);  // entropy-fix wave speeds (lamE[0], lamE[2])
 
   {
-    double eL = u1 * nx + v1 * ny - kSqrtG * sqh1;
-    double eR = u2 * nx + v2 * ny - kSqrtG * sqh2;
+    T eL = u1 * nx + v1 * ny - kSqrtG * sqh1;
+    T eR = u2 * nx + v2 * ny - kSqrtG * sqh2;
     if (eL < 0.0 && eR > 0.0) {
-      double split = eL * (eR - l0) / (eR - eL);
+      T split = eL * (eR - l0) / (eR - eL);
       e0 = l0 - split;
       l0 = split;
     }
     eL = u1 * nx + v1 * ny + kSqrtG * sqh1;
     eR = u2 * nx + v2 * ny + kSqrtG * sqh2;
     if (eL < 0.0 && eR > 0.0) {
-      double split = eR * (l2 - eL) / (eR - eL);
+      T split = eR * (l2 - eL) / (eR - eL);
       e2 = l2 - split;
       l2 = split;
     }
   }
 
   // eigenvectors (row i = wave i)
-  const double ev01 = ubar - c * nx, ev02 = vbar - c * ny;
-  const double ev11 = -c * ny, ev12 = c * nx;
-  const double ev21 = ubar + c * nx, ev22 = vbar + c * ny;
+  const T ev01 = ubar - c * nx, ev02 = vbar - c * ny;
+  const T ev11 = -c * ny, ev12 = c * nx;
+  const T ev21 = ubar + c * nx, ev22 = vbar + c * ny;
 
-  const double dqx = qx2 - qx1, dqy = qy2 - qy1;
-  const double a0 = 0.5 * (dh - ((dqx * nx + dqy * ny) - un * dh) / c);
-  const double a1 = ((dqy - vbar * dh) * nx - (dqx - ubar * dh) * ny) / c;
-  const double a2 = 0.5 * (dh + ((dqx * nx + dqy * ny) - un * dh) / c);
+  const T dqx = qx2 - qx1, dqy = qy2 - qy1;
+  const T a0 = 0.5 * (dh - ((dqx * nx + dqy * ny) - un * dh) / c);
+  const T a1 = ((dqy - vbar * dh) * nx - (dqx - ubar * dh) * ny) / c;
+  const T a2 = 0.5 * (dh + ((dqx * nx + dqy * ny) - un * dh) / c);
 
-  const double dz = z2 - z1;
-  double dzp = dz, hp;
+  const T dz = z2 - z1;
+  T dzp = dz, hp;
   if (dz >= 0.0) {
     hp = h1;
     if (z1 + h1 < z2) dzp = h1;
@@ -88,94 +129,105 @@ __device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, d
     hp = h2;
     if (z2 + h2 < z1) dzp = -h2;
   }
-  double b0 = 0.5 * kG * (hp - 0.5 * fabs(dzp)) * dzp / c;
+  T b0 = 0.5 * kG * (hp - 0.5 * fabs(dzp)) * dzp / c;
 
   hp = h1 + a0;
   if (l0 * l2 < 0.0 && hp > 0.0 && h1 > 0.0 && h2 > 0.0) {
-    b0 = fmax(b0, l0 / (l0 + e2) * (a0 * l0 - c1.hd + e2 * h2));
-    b0 = fmin(b0, l2 / (l2 + e0) * (-a2 * l2 + c2.hd + e0 * h1));
+    b0 = tmax<T>(b0, l0 / (l0 + e2) * (a0 * l0 - c1.hd + e2 * h2));
+    b0 = tmin<T>(b0, l2 / (l2 + e0) * (-a2 * l2 + c2.hd + e0 * h1));
   }
-  const double b2 = -b0;
+  const T b2 = -b0;
 
-  double d10 = 0.0, d11 = 0.0, d12 = 0.0, d20 = 0.0, d21 = 0.0, d22 = 0.0;
-  const double sL = hp - b0 / l0;
-  const double sR = hp + b2 / l2;
-  if (sR < -kEps12 && h2 < kEps12) {
+  const T one = T(1), zero = T(0);
+  T d10 = zero, d11 = zero, d12 = zero, d20 = zero, d21 = zero, d22 = zero;
+  const T sL = hp - b0 / l0;
+  const T sR = hp + b2 / l2;
+  if (sR < -eps && h2 < eps) {
     d10 += (l0 * a0 - b0) + (l2 * a2 - b2);
-  } else if (sL < -kEps12 && h1 < kEps12) {
+  } else if (sL < -eps && h1 < eps) {
     d20 += (l0 * a0 - b0) + (l2 * a2 - b2);
   } else {
     // wave 0: eigenvector (1, ev01, ev02)
-    double w = l0 * a0 - b0;
-    if (l0 > 0.0) { d20 += w * 1.0; d21 += w * ev01; d22 += w * ev02; }
-    else          { d10 += w * 1.0; d11 += w * ev01; d12 += w * ev02; }
+    T w = l0 * a0 - b0;
+    if (l0 > 0.0) { d20 += w * one; d21 += w * ev01; d22 += w * ev02; }
+    else          { d10 += w * one; d11 += w * ev01; d12 += w * ev02; }
     // wave 1: eigenvector (0, ev11, ev12), beta1 = 0
-    w = l1 * a1 - 0.0;
-    if (l1 > 0.0) { d20 += w * 0.0; d21 += w * ev11; d22 += w * ev12; }
-    else          { d10 += w * 0.0; d11 += w * ev11; d12 += w * ev12; }
+    w = l1 * a1 - zero;
+    if (l1 > 0.0) { d20 += w * zero; d21 += w * ev11; d22 += w * ev12; }
+    else          { d10 += w * zero; d11 += w * ev11; d12 += w * ev12; }
     // wave 2: eigenvector (1, ev21, ev22)
     w = l2 * a2 - b2;
-    if (l2 > 0.0) { d20 += w * 1.0; d21 += w * ev21; d22 += w * ev22; }
-    else          { d10 += w * 1.0; d11 += w * ev21; d12 += w * ev22; }
+    if (l2 > 0.0) { d20 += w * one; d21 += w * ev21; d22 += w * ev22; }
+    else          { d10 += w * one; d11 += w * ev21; d12 += w * ev22; }
   }
   // entropy-fix parts; lamE[1] is always 0
   {
-    double w = e0 * a0;
-    if (e0 > 0.0) { d20 += w * 1.0; d21 += w * ev01; d22 += w * ev02; }
-    else          { d10 += w * 1.0; d11 += w * ev01; d12 += w * ev02; }
-    w = 0.0 * a1;
-    d10 += w * 0.0; d11 += w * ev11; d12 += w * ev12;
+    T w = e0 * a0;
+    if (e0 > 0.0) { d20 += w * one; d21 += w * ev01; d22 += w * ev02; }
+    else          { d10 += w * one; d11 += w * ev01; d12 += w * ev02; }
+    w = zero * a1;
+    d10 += w * zero; d11 += w * ev11; d12 += w * ev12;
     w = e2 * a2;
-    if (e2 > 0.0) { d20 += w * 1.0; d21 += w * ev21; d22 += w * ev22; }
-    else          { d10 += w * 1.0; d11 += w * ev21; d12 += w * ev22; }
+    if (e2 > 0.0) { d20 += w * one; d21 += w * ev21; d22 += w * ev22; }
+    else          { d10 += w * one; d11 += w * ev21; d12 += w * ev22; }
   }
-  f.d1[0] = fabs(d10) < kEps12 ? 0.0 : d10;
-  f.d1[1] = fabs(d11) < kEps12 ? 0.0 : d11;
-  f.d1[2] = fabs(d12) < kEps12 ? 0.0 : d12;
-  f.d2[0] = fabs(d20) < kEps12 ? 0.0 : d20;
-  f.d2[1] = fabs(d21) < kEps12 ? 0.0 : d21;
-  f.d2[2] = fabs(d22) < kEps12 ? 0.0 : d22;
+  f.d1[0] = fabs(d10) < eps ? zero : d10;
+  f.d1[1] = fabs(d11) < eps ? zero : d11;
+  f.d1[2] = fabs(d12) < eps ? zero : d12;
+  f.d2[0] = fabs(d20) < eps ? zero : d20;
+  f.d2[1] = fabs(d21) < eps ? zero : d21;
+  f.d2[2] = fabs(d22) < eps ? zero : d22;
   return true;
 }
 
-// The face with the reference's per-face argument set (the drop-in kernels).
 template <int NX, int NY>
 __device__ __forceinline__ bool roe_face_f64(double h1, double h2, double qx1, double qx2,
-                                             double qy1, double qy2, double z1, double z2,
-                                             double sqh1, double sqh2, double dx, double dt,
-                                             double hextra, Face64& f) {
-  return roe_face_f64<NX, NY>(h1, h2, qx1, qx2, qy1, qy2, cell64(h1, qx1, qy1, dx, dt, hextra),
-                              cell64(h2, qx2, qy2, dx, dt, hextra), z1, z2, sqh1, sqh2, f);
+                                             double qy1, double qy2, const Cell64& c1,
+                                             const Cell64& c2, double z1, double z2,
+                                             double sqh1, double sqh2, Face64& f) {
+  return roe_face_t<double, NX, NY>(h1, h2, qx1, qx2, qy1, qy2, c1, c2, z1, z2, sqh1, sqh2, f);
+}
+
+// The face with the reference's per-face argument set (the drop-in kernels).
+template <class T, int NX, int NY>
+__device__ __forceinline__ bool roe_face_ref(T h1, T h2, T qx1, T qx2, T qy1, T qy2, T z1, T z2,
+                                             T sqh1, T sqh2, T dx, T dt, T hextra, FaceT<T>& f) {
+  return roe_face_t<T, NX, NY>(h1, h2, qx1, qx2, qy1, qy2, cell_t<T>(h1, qx1, qy1, dx, dt, hextra),
+                               cell_t<T>(h2, qx2, qy2, dx, dt, hextra), z1, z2, sqh1, sqh2, f);
 }
 
 // src/kernels.h:176-185 -- per-cell CFL step (MAX_VALUE when shallower than
-// hextra) and the sqrt(h) cache value.
-__device__ __forceinline__ double cell_dt_f64(double h, double qx, double qy, double sq,
-                                              double cn_dx, double hextra) {
+// hextra) and the sqrt(h) cache value; cn_dx is the reference's cn*dx in T.
+template <class T>
+__device__ __forceinline__ T cell_dt_t(T h, T qx, T qy, T sq, T cn_dx, T hextra) {
   if (h > hextra) {
-    double umax = fmax(fabs(qx), fabs(qy)) / h;
+    const T umax = tmax<T>(fabs(qx), fabs(qy)) / h;
     return cn_dx / (umax + kSqrtG * sq);
   }
-  return kMaxDt;
+  return T(kMaxDt);
+}
+__device__ __forceinline__ double cell_dt_f64(double h, double qx, double qy, double sq,
+                                              double cn_dx, double hextra) {
+  return cell_dt_t<double>(h, qx, qy, sq, cn_dx, hextra);
 }
 
 // src/kernels_state.h:62-101 -- new depth and discharge from the RHS sums.
-__device__ __forceinline__ void update_cell_f64(double hn, double qxn, double qyn, double n2,
-                                                double rh0, double rh1, double rqx0,
-                                                double rqx1, double rqy0, double rqy1,
-                                                double dt, double hextra, double& h_out,
-                                                double& qx_out, double& qy_out) {
-  double hnew = hn - rh0 - rh1;
-  if (hnew < 0.0) hnew = 0.0;
+template <class T>
+__device__ __forceinline__ void update_cell_t(T hn, T qxn, T qyn, T n2, T rh0, T rh1, T rqx0, T rqx1,
+                                              T rqy0, T rqy1, T dt, T hextra, T& h_out, T& qx_out,
+                                              T& qy_out) {
+  constexpr double eps = eps12_of<T>();
+  T hnew = hn - rh0 - rh1;
+  if (hnew < 0.0) hnew = T(0);
   if (hnew < hextra) {
-    qx_out = 0.0;
-    qy_out = 0.0;
+    qx_out = T(0);
+    qy_out = T(0);
   } else {
-    double mx = qxn - rqx0 - rqx1;
-    double my = qyn - rqy0 - rqy1;
-    double modm = sqrt(mx * mx + my * my);
-    if (n2 > kEps12 && hn >= hextra && modm > kEps12) {
-      double tt = dt * kG * n2 * modm / (hn * hn * cbrt(hn));
+    T mx = qxn - rqx0 - rqx1;
+    T my = qyn - rqy0 - rqy1;
+    T modm = sqrt(double(mx * mx + my * my));
+    if (n2 > eps && hn >= hextra && modm > eps) {
+      T tt = dt * kG * n2 * modm / (hn * hn * cbrt(double(hn)));
       qx_out = -0.5 * (mx - mx * sqrt(1.0 + 4.0 * tt)) / tt;
       qy_out = -0.5 * (my - my * sqrt(1.0 + 4.0 * tt)) / tt;
     } else {
@@ -184,6 +236,14 @@ __device__ __forceinline__ void update_cell_f64(double hn, double qxn, double qy
     }
   }
   h_out = hnew;
+}
+__device__ __forceinline__ void update_cell_f64(double hn, double qxn, double qyn, double n2,
+                                                double rh0, double rh1, double rqx0,
+                                                double rqx1, double rqy0, double rqy1,
+                                                double dt, double hextra, double& h_out,
+                                                double& qx_out, double& qy_out) {
+  update_cell_t<double>(hn, qxn, qyn, n2, rh0, rh1, rqx0, rqx1, rqy0, rqy1, dt, hextra, h_out, qx_out,
+                        qy_out);
 }
 
 }  // namespace tr

@@ -820,6 +820,24 @@ int tr_sim_set_obs_halt(tr_sim* s, int32_t on) {
   return TR_OK;
 }
 
+// Rows per wave of the FP64 fused step for the launches queued next (0: by
+// grid size); results do not depend on it.
+int tr_sim_set_strip_rows(tr_sim* s, int32_t strip_rows) {
+  if (!s || strip_rows < 0 || strip_rows > 4096) {
+    set_error("tr_sim_set_strip_rows: strip_rows must be in [0, 4096]");
+    return TR_ERR_INVALID;
+  }
+  if (s->form != TR_FORM_F64_FUSED && s->form != TR_FORM_F64_R32) {
+    set_error("tr_sim_set_strip_rows: only the fused FP64 forms have strips");
+    return TR_ERR_STATE;
+  }
+  TR_HIP(hipStreamSynchronize(s->stream));
+  s->strip_rows = strip_rows > 0 ? strip_rows : tr_default_strip_rows(s->v.rows, s->v.cols);
+  return TR_OK;
+}
+
+int32_t tr_sim_strip_rows(const tr_sim* s) { return s ? s->strip_rows : 0; }
+
 void* tr_sim_stream(tr_sim* s) {
   if (!s) return nullptr;
   if (s->batch) return tr_batch_stream(s->batch);

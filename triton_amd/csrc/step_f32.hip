@@ -27,6 +27,7 @@
 
 #include "batch.hpp"
 #include "common.hpp"
+#include "sround.hpp"
 #include "step_ctrl.hpp"
 
 #pragma clang fp contract(fast)
@@ -50,59 +51,12 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float rsqf(float x) { return __builtin_amdgcn_rsqf(fmaxf(x, 1e-30f)); }
 __device__ __forceinline__ float sqrtf_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// a + b == s + lo exactly (Knuth's TwoSum), s returned.
-__device__ __forceinline__ float two_sum(float a, float b, float& lo) {
-#pragma clang fp contract(off)
-  const float s = a + b;
-  const float bb = s - a;
-  lo = (a - (s - bb)) + (b - bb);
-  return s;
-}
-
 // The reference's h < 0 -> 0 on the two-float depth hs + he (hs == 0 with
 // he < 0 is left to the final clamp of the rounded depth).
 __device__ __forceinline__ void clamp_neg(float& hs, float& he) {
   const bool neg = hs < 0.f;
   hs = neg ? 0.f : hs;
   he = neg ? 0.f : he;
-}
-
-// Stored depths are rounded stochastically.  With round-to-nearest the FP32
-// form lost mass steadily on deep water, where a step changes a cell by a few
-// ulps (an ulp is 9.5e-7 m at 10 m): -1.3e-5 relative mass over two periods
-// of the 1024^2 Thacker bowl, +8.6e-6 at 512^2, linear in the step count, the
-// same with every dry threshold and with IEEE division, square root and no
-// FMA contraction; one FP32 step from an FP64 state showed no such loss, one
-// from an FP32-evolved state did.  Rounded stochastically: +1.2e-9 and
-// -2.9e-9.  The stored depth is hs + he plus a dither uniform over one ulp of
-// hs, rounded to nearest: it lands on the float one ulp away in he's
-// direction with probability |he| / ulp (|he| <= ulp/2), else on hs, so its
-// expected value is hs + he (a dither as wide as a grid spacing keeps this
-// where the spacing halves below a power of two).  The dither is x 2^-32 ulp,
-// x = (column term) + fmix32(row and iteration term): hashed, so a cell's
-// dither at the next step is unrelated to its neighbours' dithers now (with
-// a dither linear in column, row and step, a Weyl sequence, the loss was
-// still -5.6e-6: the neighbours' roundings feed the cell's next residual).
-// It depends only on the member's cell and iteration count, so a member's
-// bits do not depend on the batch or the partition, and a hotstart continues
-// the same sequence.  (Below 2^-71 m the scaled ulp underflows and the depth
-// is rounded to nearest.)
-__device__ __forceinline__ float sround(float hs, float he, int x) {
-  const float ulp_2m32 = __int_as_float(__float_as_int(hs) & 0x7f800000) * 0x1p-55f;
-  return fmaxf(hs + fmaf(float(x), ulp_2m32, he), 0.f);
-}
-constexpr unsigned kDitherCol = 0x85EBCA77u;
-constexpr unsigned kDitherRow = 0xC2B2AE3Du;
-constexpr unsigned kDitherStep = 0x9E3779B9u;  // 2^32 / golden ratio
-// MurmurHash3's 32-bit finaliser: the row-and-iteration part of the dither
-// (wave-uniform, so it runs on the scalar unit)
-__device__ __forceinline__ unsigned fmix32(unsigned h) {
-  h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  h *= 0xC2B2AE35u;
-  h ^= h >> 16;
-  return h;
 }
 
 struct Side {
@@ -363,6 +317,8 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
 
   const int c = kOutCols * bx - 1 + lane;
   const unsigned dither_col = unsigned(c) * kDitherCol;
+  const unsigned dither_colq = unsigned(c) * kDitherRow;
+  const bool round_q = P.round_q != 0;
   const unsigned col4 = unsigned(min(max(c, 0), Cc - 1)) * 4u;
   const bool col_int = c >= 1 && c <= Cc - 2;
   const bool out_lane = lane >= 2 && lane <= kOutCols + 1 && col_int;
@@ -424,8 +380,15 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
         he = e - de;
       }
       clamp_neg(hs, he);
-      const float mx = pqx - dtdx * (p0x + (p1x + y1x));
-      const float my = pqy - dtdx * (p0y + (p1y + y1y));
+      float mxe = 0.f, mye = 0.f;
+      float mx, my;
+      if (round_q) {
+        mx = sub_scaled2(pqx, dtdx, p0x + (p1x + y1x), mxe);
+        my = sub_scaled2(pqy, dtdx, p0y + (p1y + y1y), mye);
+      } else {
+        mx = pqx - dtdx * (p0x + (p1x + y1x));
+        my = pqy - dtdx * (p0y + (p1y + y1y));
+      }
       float kf = 1.f;
       if (pn2 > eps && ph >= hextra) {
         const float m2 = mx * mx + my * my;
@@ -437,8 +400,18 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
         kf = mm > eps ? k : 1.f;
       }
       const bool wet = hs >= hextra;
-      nqx = wet ? mx * kf : 0.f;
-      nqy = wet ? my * kf : 0.f;
+      if (round_q) {
+        // (ms + me) kf carried as qs + qe, then rounded stochastically
+        const unsigned hrow = dither_step + unsigned(ru) * kDitherRow;
+        const float qxs = mx * kf, qys = my * kf;
+        const float qxe = fmaf(mx, kf, -qxs) + mxe * kf;
+        const float qye = fmaf(my, kf, -qys) + mye * kf;
+        nqx = wet ? sround_signed(qxs, qxe, int(dither_colq + fmix32(hrow ^ kDitherQx))) : 0.f;
+        nqy = wet ? sround_signed(qys, qye, int(dither_colq + fmix32(hrow ^ kDitherQy))) : 0.f;
+      } else {
+        nqx = wet ? mx * kf : 0.f;
+        nqy = wet ? my * kf : 0.f;
+      }
       if (L.nz > 0) {
         float e;
         hs = two_sum(hs, padd, e);

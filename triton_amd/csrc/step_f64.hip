@@ -17,6 +17,7 @@
 
 #include "common.hpp"
 #include "roe_f64.hpp"
+#include "sround.hpp"
 #include "step.hpp"
 #include "step_ctrl.hpp"
 
@@ -49,8 +50,11 @@ __global__ __launch_bounds__(64) void k_close_f64(StepParams<double> p, int afte
 }
 
 // kR32: every updated h, qx, qy is rounded to FP32 before it is used or
-// stored (FP64 arithmetic on FP32-stored state: the accuracy yardstick of
-// the FP32 form, TR_FORM_F64_R32).
+// stored, by the FP32 form's storage rule: the depth after the update and
+// the rain stochastically with the FP32 form's dither (sround.hpp), then the
+// point sources and the discharges to nearest.  FP64 arithmetic on
+// FP32-stored state: the accuracy yardstick of the FP32 form
+// (TR_FORM_F64_R32).
 template <bool kOpen, bool kBC, bool kR32>
 __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const int lane = threadIdx.x;
@@ -60,6 +64,7 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   const double simtime = cc->simtime;
   const int roff_row = cc->roff_row;
   const double* __restrict__ src_add = p.src_add + p.cur * p.nsrc;
+  const unsigned dither_step = unsigned(p.it_count0 + cc->step) * kDitherStep;
   const double dx = p.dx, hextra = p.hextra;
   const int R = p.rows, C = p.cols;
   const int tile = xcd_slot(int(blockIdx.x), int(p.nblocks));
@@ -169,6 +174,8 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
         nh = hc;
       }
     }
+    if (kR32 && ru_int && col_int)
+      nh = double(sround_f64(nh, int(unsigned(c) * kDitherCol + fmix32(dither_step + unsigned(ru) * kDitherRow))));
     if (ru_int) {
       for (int k = 0; k < p.nsrc; k++) {
         if (p.src_row[k] == ru && p.src_col[k] == c) nh = nh + src_add[k];

@@ -41,6 +41,10 @@ class ServeRequest(C.Structure):
                 ("admitted", C.c_double), ("done", C.c_double), ("status", SimStatus)]
 
 
+class ServeRequestOptions(C.Structure):
+    _fields_ = [("output_dir", C.c_char_p), ("output_format", C.c_int32), ("keep_prints", C.c_int32)]
+
+
 class ServeStatsC(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("waiting", C.c_int64),
                 ("running", C.c_int64), ("cell_steps", C.c_int64), ("rounds", C.c_int64),
@@ -63,6 +67,11 @@ def _serve_lib() -> C.CDLL:
             "tr_serve_poll": (C.c_int, [vp, i32, C.POINTER(ServeRequest)]),
             "tr_serve_result": (C.c_int, [vp, i32] + [C.POINTER(C.c_double)] * 3),
             "tr_serve_release": (C.c_int, [vp, i32]),
+            "tr_serve_submit_ex": (C.c_int, [vp, C.POINTER(CaseView), f64, C.POINTER(ServeRequestOptions),
+                                             C.POINTER(i32)]),
+            "tr_serve_print_count": (C.c_int, [vp, i32, C.POINTER(i32)]),
+            "tr_serve_print": (C.c_int, [vp, i32, i32, C.POINTER(i32), C.POINTER(f64)]
+                               + [C.POINTER(C.c_double)] * 4),
             "tr_serve_cancel": (C.c_int, [vp, i32]),
             "tr_serve_stats_get": (C.c_int, [vp, C.POINTER(ServeStatsC)]),
             "tr_serve_destroy": (None, [vp]),
@@ -110,6 +119,7 @@ class Request:
     status: dict | None = None
     state: dict | None = None
     tag: object = None
+    prints: list = field(default_factory=list)  # (print_id, simtime, state) kept at print times
 
     @property
     def latency(self) -> float | None:
@@ -178,12 +188,18 @@ class Server:
         self.close()
 
     # -- client side -------------------------------------------------------
-    def submit(self, case: Case, tag=None, arrival: float | None = None) -> int:
-        """Queue a request; `arrival` is on the server's clock (default now)."""
+    def submit(self, case: Case, tag=None, arrival: float | None = None, output_dir=None,
+               output_format: str = "ASC", keep_prints: bool = False) -> int:
+        """Queue a request; `arrival` is on the server's clock (default now).
+        output_dir: write the reference's print rasters there at every print
+        time (output_format "ASC": ASCII and binary twins, "BIN": binary);
+        keep_prints: keep each print's state (Request.prints)."""
         v = case.view()
         rid = C.c_int32()
-        check(_serve_lib().tr_serve_submit(self._h, C.byref(v), -1.0 if arrival is None else float(arrival),
-                                           C.byref(rid)))
+        ro = ServeRequestOptions(None if output_dir is None else str(output_dir).encode(),
+                                 0 if str(output_format).upper() == "ASC" else 1, int(bool(keep_prints)))
+        check(_serve_lib().tr_serve_submit_ex(self._h, C.byref(v), -1.0 if arrival is None else float(arrival),
+                                              C.byref(ro), C.byref(rid)))
         r = rid.value
         self._cases[r], self._views[r], self._tags[r] = case, v, tag
         return r
@@ -235,8 +251,22 @@ class Server:
                 P = C.POINTER(C.c_double)
                 check(L.tr_serve_result(self._h, int(r), *(state[k].ctypes.data_as(P) for k in ("h", "qx", "qy"))))
             status = p.status.as_dict()
+            prints = []
+            n = C.c_int32()
+            check(L.tr_serve_print_count(self._h, int(r), C.byref(n)))
+            P = C.POINTER(C.c_double)
+            for k in range(n.value):
+                shape = (case.rows, case.cols)
+                ps = {f: np.empty(shape, np.float64) for f in ("h", "qx", "qy", "maxh")}
+                pid, t = C.c_int32(), C.c_double()
+                check(L.tr_serve_print(self._h, int(r), k, C.byref(pid), C.byref(t),
+                                       *(ps[f].ctypes.data_as(P) for f in ("h", "qx", "qy", "maxh"))))
+                if "maxh" not in case.arrays:
+                    ps.pop("maxh")
+                prints.append((pid.value, t.value, ps))
             self._done[r] = Request(r, p.arrival, p.admitted, p.done, int(p.bytes), int(p.cells),
-                                    int(status["steps"]), int(p.skipped), status, state, self._tags.pop(r, None))
+                                    int(status["steps"]), int(p.skipped), status, state, self._tags.pop(r, None),
+                                    prints)
             self._latencies.append(p.done - p.arrival)
             check(L.tr_serve_release(self._h, int(r)))
 

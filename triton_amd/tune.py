@@ -1,11 +1,14 @@
-"""Autotuner of the batched launch shape.
+"""Autotuner of the launch shapes.
 
-The batched FP32 step has two knobs that change speed but not results:
-the number of waves per SIMD the row partition is cut into, and how many
-steps are captured per hipGraph.  autotune_batch() times each candidate on
-the batch's own members (real steps, nothing is thrown away), keeps the
-fastest and remembers it per (device, member count, total cells) bucket in a
-small JSON cache ($TR_TUNE_CACHE, default ~/.cache/triton_amd/tune.json).
+The batched FP32 step has two knobs that change speed but not results: the
+number of waves per SIMD the row partition is cut into (which sets the band
+height of each wave's tile: the smallest height whose band-strip tiles need
+no more waves than that), and how many steps are captured per hipGraph.
+The fused FP64 step has one: the rows per wave of its strips.
+autotune_batch() and autotune_sim() time each candidate on the work itself
+(real steps, nothing is thrown away), keep the fastest and remember it per
+(device, member count, total cells) bucket in a small JSON cache
+($TR_TUNE_CACHE, default ~/.cache/triton_amd/tune.json).
 """
 from __future__ import annotations
 
@@ -14,14 +17,18 @@ import math
 import os
 from pathlib import Path
 
-# (waves per SIMD, steps per graph).  About 5 waves per SIMD are resident at
+# (waves per SIMD, steps per graph).  About 6 waves per SIMD are resident at
 # once (the step kernel's registers); more queue behind them, which evens out
 # uneven pieces (an ensemble of mixed sizes: on the 64-member ensemble 10,
 # 15, 20, 25 and 30 waves per SIMD measured 94.8, 98.5, 100.5, 100.4 and
 # 99.8 GCUPS) but costs L2 sharing between neighbouring bands when the
 # pieces are equal (one large grid, where 5 is fastest).  hipGraph replays
-# measured the same as eager launches, so the candidates do not use them.
-DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0), (20, 0), (25, 0))
+# of 10 steps are candidates too: they save the launch gaps, which matter
+# when a step is short (small batches) and little otherwise (within 1 % on
+# the ensemble).
+DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0), (20, 0), (25, 0), (5, 10), (15, 10), (20, 10), (25, 10))
+# rows per wave of the fused FP64 step
+DEFAULT_STRIPS = (16, 32, 48, 64, 96, 128)
 
 
 def cache_path() -> Path:
@@ -47,11 +54,14 @@ def _store(table: dict) -> None:
         pass  # a read-only home only costs the re-tune
 
 
-def bucket(batch) -> str:
+def _device() -> str:
     import torch
+    return torch.cuda.get_device_name(torch.cuda.current_device()) if torch.cuda.is_available() else "none"
+
+
+def bucket(batch) -> str:
     cells = sum((c.rows - 2) * (c.cols - 2) for c in batch.members.values())
-    dev = torch.cuda.get_device_name(torch.cuda.current_device()) if torch.cuda.is_available() else "none"
-    return f"{dev}|m{len(batch.members)}|c2^{round(math.log2(max(cells, 1)))}"
+    return f"{_device()}|m{len(batch.members)}|c2^{round(math.log2(max(cells, 1)))}"
 
 
 def time_steps(batch, steps: int) -> float:
@@ -90,5 +100,45 @@ def autotune_batch(batch, candidates=None, steps: int = 8, cache: bool = True) -
     batch.set_launch(*best)
     if cache:
         table[key] = {"launch": list(best), "ms": {f"{w},{g}": round(t, 5) for t, (w, g) in results}}
+        _store(table)
+    return best
+
+
+def time_sim_steps(sim, steps: int) -> float:
+    """Milliseconds per step of a fused simulation over `steps` steps (after
+    as many untimed ones)."""
+    import torch
+    st = torch.cuda.ExternalStream(sim.stream)
+    sim.arm(max_steps=2 * steps)
+    sim.enqueue(steps)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    sim.enqueue(steps)
+    e1.record(st)
+    e1.synchronize()
+    sim.sync()
+    return e0.elapsed_time(e1) / steps
+
+
+def autotune_sim(sim, candidates=None, steps: int = 8, cache: bool = True) -> int:
+    """Pick the strip height (rows per wave) of a fused FP64 simulation;
+    returns it.  The simulation advances by 2 * steps * len(candidates) steps
+    at most."""
+    c = sim.case
+    key = f"{_device()}|f64|{c.rows}x{c.cols}"
+    table = _load() if cache else {}
+    if key in table:
+        rows = int(table[key]["strip_rows"])
+        sim.set_strip_rows(rows)
+        return rows
+    results = []
+    for rows in (candidates or DEFAULT_STRIPS):
+        sim.set_strip_rows(rows)
+        results.append((time_sim_steps(sim, steps), int(rows)))
+    best = min(results)[1]
+    sim.set_strip_rows(best)
+    if cache:
+        table[key] = {"strip_rows": best, "ms": {str(r): round(t, 5) for t, r in results}}
         _store(table)
     return best
