@@ -174,6 +174,7 @@ Config config_from_text(const std::string& text, const std::string& base_dir) {
   c.hydrograph_filename = resolve(get(m, "hydrograph_filename"), base_dir);
   c.runoff_filename = resolve(get(m, "runoff_filename"), base_dir);
   c.dem_filename = resolve(get(m, "dem_filename"), base_dir);
+  c.header_filename = resolve(get(m, "header_filename"), base_dir);
   c.src_loc_file = resolve(get(m, "src_loc_file"), base_dir);
   c.runoff_map = resolve(get(m, "runoff_map"), base_dir);
   c.observation_loc_file = resolve(get(m, "observation_loc_file"), base_dir);
@@ -608,45 +609,58 @@ Case load_case(const std::string& cfg_path, const std::string& base_dir) {
   Case c;
   c.cfg = load_config(cfg_path, base_dir);
   const Config& cfg = c.cfg;
-  if (cfg.input_option != "SEQ") throw InputError("only input_option=SEQ is supported");
+  // input_option=PAR (src/triton_init.h:397-563) reads rasters split per
+  // rank as <name>_<rank>.<ext>, the DEM header from header_filename (ASCII
+  // whatever the input format) and the DEM itself without it.  One GPU holds
+  // the whole grid, so it reads the one-partition set, rank 00.
+  const bool par = cfg.input_option == "PAR";
+  if (!par && cfg.input_option != "SEQ") throw InputError("input_option must be SEQ or PAR");
   if (cfg.dem_filename.empty()) throw InputError("dem_filename is required");
+  if (par && cfg.header_filename.empty()) throw InputError("input_option=PAR needs header_filename");
+  auto part = [&](const std::string& name, const char* ext) { return par ? name + "_00" + ext : name; };
   const bool asc = cfg.input_format == "ASC";
-  c.dem_hdr = asc ? read_dem_header_ascii(cfg.dem_filename) : read_dem_header_binary(cfg.dem_filename);
+  if (par) c.dem_hdr = read_dem_header_ascii(cfg.header_filename);
+  else c.dem_hdr = asc ? read_dem_header_ascii(cfg.dem_filename) : read_dem_header_binary(cfg.dem_filename);
   const int nr = c.dem_hdr.nrows, nc = c.dem_hdr.ncols;
-  if (nr <= 0 || nc <= 0) throw InputError("bad DEM header in " + cfg.dem_filename);
-  std::vector<double> dem = asc ? read_ascii_raster(cfg.dem_filename, nr, nc, 6, false)
-                                : read_binary_raster(cfg.dem_filename, nr, nc, 6, false);
-  std::vector<double> nin, hin, qxin, qyin;
+  if (nr <= 0 || nc <= 0) throw InputError("bad DEM header in " + (par ? cfg.header_filename : cfg.dem_filename));
+  const std::string dem_path = part(cfg.dem_filename, ".dem");
+  std::vector<double> dem;
+  if (par) dem = asc ? read_ascii_raster(dem_path, nr, nc, 0, false) : read_binary_raster(dem_path, nr, nc, 2, true);
+  else dem = asc ? read_ascii_raster(dem_path, nr, nc, 6, false) : read_binary_raster(dem_path, nr, nc, 6, false);
+  std::vector<double> nin, hin, qxin, qyin, mhin;
   std::vector<int32_t> rmap;
-  if (!cfg.n_infile.empty()) nin = load_raster(cfg, cfg.n_infile, nr, nc);
+  if (!cfg.n_infile.empty()) nin = load_raster(cfg, part(cfg.n_infile, ".mann"), nr, nc);
   if (cfg.checkpoint_id > 0) {
-    // hotstart from the BIN rasters of a previous print (src/triton_init.h:303-353)
+    // hotstart from the BIN rasters of a previous print (src/triton_init.h:303-353,
+    // :576-631), the running maximum too when it is printed
     char tag[32];
     std::snprintf(tag, sizeof(tag), "%02d_00.out", cfg.checkpoint_id);
     std::string bin = resolve(cfg.output_folder, base_dir) + "/bin/";
     hin = read_binary_raster(bin + "H_" + tag, nr, nc, 2, true);
     qxin = read_binary_raster(bin + "QX_" + tag, nr, nc, 2, true);
     qyin = read_binary_raster(bin + "QY_" + tag, nr, nc, 2, true);
+    if (!cfg.max_value_print_option.empty()) mhin = read_binary_raster(bin + "MH_" + tag, nr, nc, 2, true);
   } else {
-    if (!cfg.h_infile.empty()) hin = load_raster(cfg, cfg.h_infile, nr, nc);
-    if (!cfg.qx_infile.empty()) qxin = load_raster(cfg, cfg.qx_infile, nr, nc);
-    if (!cfg.qy_infile.empty()) qyin = load_raster(cfg, cfg.qy_infile, nr, nc);
+    if (!cfg.h_infile.empty()) hin = load_raster(cfg, part(cfg.h_infile, ".out"), nr, nc);
+    if (!cfg.qx_infile.empty()) qxin = load_raster(cfg, part(cfg.qx_infile, ".out"), nr, nc);
+    if (!cfg.qy_infile.empty()) qyin = load_raster(cfg, part(cfg.qy_infile, ".out"), nr, nc);
   }
   if (!cfg.runoff_map.empty() && cfg.num_runoffs > 0) {
+    const std::string rmap_path = part(cfg.runoff_map, ".rmap");
     if (asc) {
-      auto v = read_ascii_raster(cfg.runoff_map, nr, nc, 0, true);
+      auto v = read_ascii_raster(rmap_path, nr, nc, 0, true);
       rmap.resize(v.size());
       for (size_t i = 0; i < v.size(); i++) rmap[i] = int32_t(v[i]);
     } else {
       // matrix<int> BIN: int32 header (rows, cols) then int32 data
-      std::ifstream f(cfg.runoff_map, std::ios::binary);
-      if (!f.good()) throw InputError("Error reading file: " + cfg.runoff_map);
+      std::ifstream f(rmap_path, std::ios::binary);
+      if (!f.good()) throw InputError("Error reading file: " + rmap_path);
       int32_t hdr[2] = {0, 0};
       f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
-      if (hdr[0] != nr || hdr[1] != nc) throw InputError("Invalid Matrix dimensions in " + cfg.runoff_map);
+      if (hdr[0] != nr || hdr[1] != nc) throw InputError("Invalid Matrix dimensions in " + rmap_path);
       rmap.resize(size_t(nr) * nc);
       f.read(reinterpret_cast<char*>(rmap.data()), std::streamsize(rmap.size() * sizeof(int32_t)));
-      if (!f) throw InputError("Short read in " + cfg.runoff_map);
+      if (!f) throw InputError("Short read in " + rmap_path);
     }
   }
   Table hyg, roff;
@@ -654,6 +668,10 @@ Case load_case(const std::string& cfg_path, const std::string& base_dir) {
   if (cfg.num_runoffs > 0) roff = load_table_csv(cfg.runoff_filename);
   pad_and_finalize(c, dem, nin, hin, qxin, qyin, rmap, cfg.num_sources > 0 ? &hyg : nullptr,
                    cfg.num_runoffs > 0 ? &roff : nullptr);
+  if (!mhin.empty()) {
+    c.maxh = pad(mhin, nr, nc, 0.0);
+    if (cfg.open_boundaries) copy_into_ghosts(c.maxh, c.rows, c.cols);
+  }
   return c;
 }
 

@@ -31,7 +31,7 @@ struct Config {
   std::string dem_filename, input_format, input_option, output_format, output_option;
   std::string output_folder, outfile_pattern, print_option, max_value_print_option;
   std::string hydrograph_filename, src_loc_file, runoff_filename, runoff_map;
-  std::string observation_loc_file, extbc_file, extbc_dir;
+  std::string observation_loc_file, extbc_file, extbc_dir, header_filename;
   std::string h_infile, qx_infile, qy_infile, n_infile;
   double const_mann = 0, time_step = 0, courant = 0, hextra = 0;
   double sim_start_time = 0, sim_duration = 0, print_interval = 0, print_observation = 0;
