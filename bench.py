@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="offered load of the serving leg against the measured batched GCUPS")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0, help="HBM budget of the serving leg")
+    ap.add_argument("--launch", default=None,
+                    help="ensemble: fixed launch shape 'waves_per_simd,graph_steps' instead of the autotuner "
+                         "(counter runs use the shape a timed run chose)")
     return ap.parse_args()
 
 
@@ -89,7 +92,8 @@ def interior(case) -> int:
 
 def host_topology() -> str:
     """Sockets x cores per socket, SMT state (lscpu) and the CPUs this
-    process may use."""
+    process may use (read at start-up: once the OpenMP runtime binds the
+    main thread to one place, its own affinity shows a single CPU)."""
     info = {}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
@@ -123,7 +127,7 @@ def cpu_baseline(cases, target_s):
     one = _cpu_rate(cases, max(target_s / 4.0, 1.0))
     set_num_threads(threads)
     return {"value": res["value"], "unit": "GCUPS", "cores": threads, "kind": "port",
-            "sample": res["sample"], "host": host_topology(),
+            "sample": res["sample"], "host": HOST,
             "one_thread": {"value": one["value"], "sample": one["sample"]},
             "speedup_threads": round(res["value"] / one["value"], 2) if one["value"] else None}
 
@@ -199,9 +203,14 @@ def run_ensemble(args):
     log(f"spun up to t >= {args.spinup_s:g} s in {min(base.values())}..{max(base.values())} steps "
         f"({time.perf_counter() - t0:.1f} s)")
     # the autotuner picks the launch shape on these members with real steps
-    launch = batch.autotune(steps=8, cache=False)
+    if args.launch:
+        w, g = (int(x) for x in args.launch.split(","))
+        batch.set_launch(w, g)
+        launch = (w, g)
+    else:
+        launch = batch.autotune(steps=8, cache=False)
     tuned = batch.poll(ids[0])["steps"] - base[ids[0]]
-    log(f"autotuned launch (waves per SIMD, graph steps) = {launch} in {tuned} steps")
+    log(f"launch (waves per SIMD, graph steps) = {launch}{'' if args.launch else ' (autotuned)'} in {tuned} steps")
     batch.run(max_steps=args.warmup)
     batch.arm(max_steps=args.steps)
     torch.cuda.synchronize()
@@ -231,7 +240,7 @@ def run_ensemble(args):
         "config": {"workload": "ensemble64_fbm_watersheds_256-2048", "members": len(members),
                    "interior_cells": cells, "sides": [m.rows - 2 for m in members], "horizon_h": 2.0,
                    "form": "fp32_batched (FP32 state, FP64 time and volume)", "boundaries": "open",
-                   "waves_per_launch": batch.work_waves, "autotuned_launch": list(launch),
+                   "waves_per_launch": batch.work_waves, "autotuned_launch": list(launch), "launch_fixed": bool(args.launch),
                    "tuning_steps": tuned, "spinup_s": args.spinup_s,
                    "spinup_steps_range": [min(base.values()), max(base.values())],
                    "simtime_s_range_after": [round(min(simt), 1), round(max(simt), 1)]},
@@ -271,7 +280,7 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
                 keep.append(scenarios.restart_at(m, sim.state(), st["simtime"]))
         base = int(sim.sync()["steps"])
         # rows per wave, tuned with real steps that count towards the warmup
-        strips.append(sim.autotune(steps=4, cache=False))
+        strips.append(sim.autotune(steps=6, cache=False))
         tuned = int(sim.sync()["steps"]) - base
         if warmup > tuned:
             sim.run(max_steps=warmup - tuned)
@@ -399,7 +408,12 @@ def run_single(args):
     }
 
 
+HOST = ""
+
+
 def main():
+    global HOST
+    HOST = host_topology()
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): every run of this build "

@@ -81,3 +81,28 @@ def test_autotune_changes_speed_only(gpu, tmp_path, monkeypatch):
     for (sa, ha), (sb, hb) in zip(*res):
         assert sa == sb
         np.testing.assert_array_equal(ha, hb)
+
+
+def test_fp64_strip_autotune_changes_speed_only(gpu, tmp_path, monkeypatch):
+    """The FP64 fused step's strip height (rows per wave) is tuned with real
+    steps among the candidates and the size-based default; a run that tunes
+    equals, bit for bit, a run that takes the same steps with the default."""
+    from triton_amd.tune import DEFAULT_STRIPS
+    monkeypatch.setenv("TR_TUNE_CACHE", str(tmp_path / "tune.json"))
+    case = members()[0]
+    out = []
+    for tune in (False, True):
+        with ta.Simulation(case, ta.FORM_F64_FUSED) as sim:
+            default = sim.strip_rows
+            n = len(DEFAULT_STRIPS) + (default not in DEFAULT_STRIPS)
+            if tune:
+                best = sim.autotune(steps=3)
+                assert best in DEFAULT_STRIPS or best == default
+                assert sim.strip_rows == best
+            else:
+                sim.run(max_steps=2 * 3 * n)
+            st = sim.run(max_steps=20)
+            assert st["steps"] == 2 * 3 * n + 20
+            out.append(sim.state())
+    for k in ("h", "qx", "qy"):
+        np.testing.assert_array_equal(out[0][k], out[1][k])

@@ -28,7 +28,7 @@ from pathlib import Path
 # the ensemble).
 DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0), (20, 0), (25, 0), (5, 10), (15, 10), (20, 10), (25, 10))
 # rows per wave of the fused FP64 step
-DEFAULT_STRIPS = (16, 32, 48, 64, 96, 128)
+DEFAULT_STRIPS = (4, 8, 16, 24, 32, 48, 64, 96, 128)
 
 
 def cache_path() -> Path:
@@ -122,9 +122,9 @@ def time_sim_steps(sim, steps: int) -> float:
 
 
 def autotune_sim(sim, candidates=None, steps: int = 8, cache: bool = True) -> int:
-    """Pick the strip height (rows per wave) of a fused FP64 simulation;
-    returns it.  The simulation advances by 2 * steps * len(candidates) steps
-    at most."""
+    """Pick the strip height (rows per wave) of a fused FP64 simulation
+    among the candidates and the simulation's size-based default; returns
+    it.  The simulation advances by 2 * steps per candidate at most."""
     c = sim.case
     key = f"{_device()}|f64|{c.rows}x{c.cols}"
     table = _load() if cache else {}
@@ -133,7 +133,10 @@ def autotune_sim(sim, candidates=None, steps: int = 8, cache: bool = True) -> in
         sim.set_strip_rows(rows)
         return rows
     results = []
-    for rows in (candidates or DEFAULT_STRIPS):
+    cands = list(candidates or DEFAULT_STRIPS)
+    if sim.strip_rows not in cands:
+        cands.append(sim.strip_rows)  # the size-based default competes too
+    for rows in cands:
         sim.set_strip_rows(rows)
         results.append((time_sim_steps(sim, steps), int(rows)))
     best = min(results)[1]
