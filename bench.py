@@ -314,8 +314,10 @@ def serving(args, capacity_gcups, waves_per_simd=0):
     watershed generator; continuous batching under an HBM budget.  The rate
     is set for the requested offered load from an a-priori work estimate of
     each request (interior cells x horizon / 1.2 s, the CFL dt of 1 m deep
-    water moving at 1 m/s on the 10 m grid).  The batch takes the launch
-    shape (waves per SIMD) the ensemble was tuned to."""
+    water moving at 1 m/s on the 10 m grid); the line also reports the load
+    the stream turned out to carry (achieved_load) and its work against that
+    estimate.  The batch takes the launch shape (waves per SIMD) the
+    ensemble was tuned to."""
     import numpy as np
     from triton_amd import scenarios
     from triton_amd.serve import serve_poisson
@@ -340,6 +342,11 @@ def serving(args, capacity_gcups, waves_per_simd=0):
            "p99_latency_s": round(st.percentile(99), 3), "max_latency_s": round(max(st.latencies), 3),
            "gcups_wall": round(st.cell_steps / wall / 1e9, 3),
            "gcups_busy": round(st.cell_steps / max(st.busy_s, 1e-9) / 1e9, 3),
+           # what the stream turned out to be: served cell-steps over the wall
+           # time against the batched capacity, and the served work against
+           # the a-priori estimate the rate was set from
+           "achieved_load": round(st.cell_steps / wall / 1e9 / capacity_gcups, 3),
+           "work_over_estimate": round(st.cell_steps / float(est.sum()), 3),
            "wall_s": round(wall, 2), "max_concurrent": st.max_members,
            "peak_gb": round(st.peak_bytes / 2 ** 30, 3), "rounds": st.rounds, "chunk_steps": 32,
            "waves_per_simd": waves_per_simd,

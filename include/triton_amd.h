@@ -90,7 +90,11 @@ typedef struct tr_case_view {
 } tr_case_view;
 
 /* Parse a .cfg and load every input it names (src/triton_init.h:33-81).
- * Relative paths in the .cfg are resolved against base_dir (NULL = cwd). */
+ * Relative paths in the .cfg are resolved against base_dir (NULL = cwd).
+ * input_option SEQ reads whole rasters; PAR (src/triton_init.h:397-563)
+ * reads the one-partition set <name>_00.<ext> with the DEM header from
+ * header_filename.  A hotstart (checkpoint_id > 0) reads the print's BIN
+ * rasters, MH too when max_value_print_option is set. */
 int tr_case_load(const char *cfg_path, const char *base_dir, tr_case **out);
 /* Fill `out` with pointers into the case (valid until tr_case_free). */
 int tr_case_view_get(tr_case *c, tr_case_view *out);

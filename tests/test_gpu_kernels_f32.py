@@ -137,3 +137,57 @@ def test_copy_observation_points_f32(gpu):
     d_arr, d_pos, d_out = dv.put(arr), dv.put(pos), dv.put(np.zeros(4, np.float32))
     ta.check(L.tr_copy_observation_points_f32(4, dv.ptr(d_arr), dv.ptr(d_out), dv.ptr(d_pos), None))
     assert np.array_equal(dv.get(d_out), arr[pos])
+
+
+def test_f32_dropins_replay_the_loop(gpu):
+    """The FP32 drop-ins called in the reference's step order (compute_dt_and_sqrt,
+    find_min_dt, the dt clamp to the print time, flux_x, flux_y,
+    update_cells, open boundaries, wet_dry; src/triton_simulate.h:54-134,
+    :191-346) equal the single-precision oracle's loop (orcf_run) bit for bit
+    over 120 steps of a centred dam break, walls and open boundaries, without
+    friction (no cube root)."""
+    from oracle.pyoracle import run_f32
+    from test_oracle_f32 import centred_dambreak
+    L, dv = ta.lib(), Dev()
+    for open_bc in (0, 1):
+        a = centred_dambreak(48, open_bc).arrays
+        rows, cols = a["h"].shape
+        n = rows * cols
+        dx, cn, hextra = np.float32(1.0), np.float32(0.5), np.float32(1e-3)
+        t_end, pint, init_dt, steps = np.float32(1e30), np.float32(0.75), np.float32(1.0), 120
+        oh, oqx, oqy, osteps, _ = run_f32(a["h"], a["qx"], a["qy"], a["dem"], a["nman2"], 1.0, cn=0.5,
+                                          hextra=1e-3, open_bc=bool(open_bc), t_end=1e30, print_interval=0.75,
+                                          init_dt=1.0, max_steps=steps)
+        assert osteps == steps
+        d = {k: dv.put(a[k].astype(np.float32)) for k in ("h", "qx", "qy", "dem", "nman2")}
+        sq, dtarr, dmin = dv.put(np.zeros(n, np.float32)), dv.put(np.zeros(n, np.float32)), dv.put(np.zeros(1, np.float32))
+        rhs = [dv.put(np.zeros(n, np.float32)) for _ in range(6)]
+        P = lambda t: dv.ptr(t)  # noqa: E731
+        simtime, print_id = np.float32(0.0), 0
+        for _ in range(steps):
+            ta.check(L.tr_compute_dt_and_sqrt_f32(n, dx, P(d["qx"]), P(d["qy"]), P(d["h"]), P(sq), P(dtarr), cn,
+                                                  hextra, None))
+            hm = C.c_float()
+            ta.check(L.tr_find_min_dt_f32(n, P(dtarr), P(dmin), C.byref(hm), None))
+            dt = np.float32(hm.value)
+            if float(dt) >= 1e10 - 1:
+                dt = init_dt
+            nxt = np.float32(pint * np.float32(print_id + 1))
+            if np.float32(simtime + dt) > nxt:
+                dt = np.float32(nxt - simtime)
+            for fn in (L.tr_flux_x_f32, L.tr_flux_y_f32):
+                ta.check(fn(n, rows, cols, dx, dt, P(d["h"]), P(d["qx"]), P(d["qy"]), P(d["dem"]), P(sq),
+                            *[P(t) for t in rhs], hextra, None))
+            ta.check(L.tr_update_cells_f32(n, rows, cols, dt, P(d["h"]), P(d["qx"]), P(d["qy"]), P(d["dem"]),
+                                           P(d["nman2"]), *[P(t) for t in rhs], hextra, None))
+            if open_bc:
+                ta.check(L.tr_open_bc_west_east_f32(2 * rows, rows, cols, P(d["h"]), P(d["qx"]), P(d["qy"]), None))
+                ta.check(L.tr_open_bc_north_south_f32(cols, rows, cols, P(d["h"]), P(d["qx"]), P(d["qy"]), 0, 1,
+                                                      None))
+            ta.check(L.tr_wet_dry_f32(n, rows, cols, dt, P(d["h"]), P(d["qx"]), P(d["qy"]), P(d["dem"]), None,
+                                      hextra, 1, None))
+            simtime = np.float32(simtime + dt)
+            if simtime >= nxt:
+                print_id += 1
+        for k, o in (("h", oh), ("qx", oqx), ("qy", oqy)):
+            assert np.array_equal(dv.get(d[k]), o), (open_bc, k)
