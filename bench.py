@@ -240,7 +240,8 @@ def run_ensemble(args):
         "config": {"workload": "ensemble64_fbm_watersheds_256-2048", "members": len(members),
                    "interior_cells": cells, "sides": [m.rows - 2 for m in members], "horizon_h": 2.0,
                    "form": "fp32_batched (FP32 state, FP64 time and volume)", "boundaries": "open",
-                   "waves_per_launch": batch.work_waves, "autotuned_launch": list(launch), "launch_fixed": bool(args.launch),
+                   "waves_per_launch": batch.work_waves, "tile": {"columns": 60, "band_rows": batch.band_rows},
+                   "autotuned_launch": list(launch), "launch_fixed": bool(args.launch),
                    "tuning_steps": tuned, "spinup_s": args.spinup_s,
                    "spinup_steps_range": [min(base.values()), max(base.values())],
                    "simtime_s_range_after": [round(min(simt), 1), round(max(simt), 1)]},

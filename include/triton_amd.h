@@ -391,6 +391,11 @@ int tr_batch_set_obs_halt(tr_batch *b, int32_t id, int32_t on);
 int64_t tr_batch_device_bytes(const tr_batch *b);
 int64_t tr_batch_member_cells(tr_batch *b, int32_t id);
 int32_t tr_batch_work_waves(const tr_batch *b);
+/* Rows per band of the current partition: each wave steps one column strip
+ * (60 output columns) of one band, so with tr_batch_work_waves this is the
+ * launch's tile shape (the band height follows from the waves per SIMD that
+ * tr_batch_set_launch or the autotuner chose). */
+int32_t tr_batch_band_rows(const tr_batch *b);
 void *tr_batch_stream(tr_batch *b);
 void tr_batch_destroy(tr_batch *b);
 

@@ -150,6 +150,7 @@ def lib() -> C.CDLL:
         "tr_batch_device_bytes": (i64, [vp]),
         "tr_batch_member_cells": (i64, [vp, i32]),
         "tr_batch_work_waves": (i32, [vp]),
+        "tr_batch_band_rows": (i32, [vp]),
         "tr_batch_stream": (vp, [vp]),
         "tr_batch_destroy": (None, [vp]),
         "tr_compute_dt_and_sqrt_f64": (C.c_int, [i32, f64, vp, vp, vp, vp, vp, f64, f64, vp]),
@@ -633,6 +634,11 @@ class Batch:
     @property
     def work_waves(self) -> int:
         return int(lib().tr_batch_work_waves(self._h))
+
+    @property
+    def band_rows(self) -> int:
+        """Rows per band of the current partition (tile: 60 columns x band_rows)."""
+        return int(lib().tr_batch_band_rows(self._h))
 
     @property
     def stream(self) -> int:

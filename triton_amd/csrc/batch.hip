@@ -113,6 +113,7 @@ struct tr_batch {
   int any_bc = 0;              // some member in the partition has external boundaries
   int piece_cap = 0;
   int nwork = 0;               // waves per launch
+  int band_rows = 0;           // rows per band of the current partition (the tile height)
   int npieces = 0;
   int simd_count = 1024;
   int waves_per_simd = kWavesPerSimd;
@@ -326,6 +327,7 @@ int rebuild_work(tr_batch* b) {
   }
   std::vector<Piece> pieces;
   int nw = 0;
+  int64_t band_h = 0;
   if (total > 0) {
     const int64_t want = int64_t(b->simd_count) * b->waves_per_simd;
     auto waves_for = [&](int64_t H) {
@@ -347,6 +349,7 @@ int rebuild_work(tr_batch* b) {
       else lo = H + 1;
     }
     const int64_t H = lo;
+    band_h = H;
     for (int i : order) {
       const InstF32& P = b->m[size_t(i)].inst;
       const int nb = int((P.rows + H - 1) / H);
@@ -370,6 +373,7 @@ int rebuild_work(tr_batch* b) {
     TR_HIP(hipMemcpy(b->d_members, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
   b->nmembers = int(order.size());
   b->nwork = nw;
+  b->band_rows = nw > 0 ? int(band_h) : 0;
   b->npieces = int(pieces.size());
   b->table_dirty = false;
   return TR_OK;
@@ -955,6 +959,7 @@ int64_t tr_batch_member_cells(tr_batch* b, int32_t id) {
 }
 
 int32_t tr_batch_work_waves(const tr_batch* b) { return b ? b->nwork : 0; }
+int32_t tr_batch_band_rows(const tr_batch* b) { return b ? b->band_rows : 0; }
 
 void* tr_batch_stream(tr_batch* b) { return b ? reinterpret_cast<void*>(b->stream) : nullptr; }
 
