@@ -174,6 +174,26 @@ def traffic_for(kernel: str, workload: str):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
+def copy_rate_gbs() -> float:
+    """HBM bandwidth a plain device copy reaches on this GPU (1 GiB read and
+    1 GiB written, best of five, HIP events): the practical ceiling the
+    roofline fraction is read against, beside the 8 TB/s specification."""
+    import torch
+    a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    best = float("inf")
+    for _ in range(6):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 2 ** 30 / (best / 1e3) / 1e9
+
+
 def events(stream):
     import torch
     e0 = torch.cuda.Event(enable_timing=True)
@@ -437,6 +457,10 @@ def main():
     ta.lib()  # fails loudly when the HIP library is missing
     res = run_ensemble(args) if args.workload == "ensemble" else run_single(args)
     cases = res.pop("cases")
+    copy = copy_rate_gbs()
+    res["roofline"]["copy_gbs"] = round(copy, 1)
+    res["roofline"]["frac_of_copy"] = round(res["roofline"]["achieved"] / copy, 4)
+    log(f"device copy: {copy:.0f} GB/s; the step's algorithmic traffic is {res['roofline']['frac_of_copy']:.3f} of it")
     cpu = cpu_baseline(cases, args.cpu_seconds) if args.cpu_seconds > 0 else None
     out = {
         "metric": METRIC, "value": round(res["value"], 3), "unit": "GCUPS", "n_gpus": 1,
