@@ -11,14 +11,14 @@ Over these events it departs from FP64 by up to 8.8e-3 m at wetting fronts,
 so the target of 1e-3 m over every cell is out of reach of any FP32-storage
 form there; asserted per event when it is so (printed either way).
 Asserted for the FP32 form:
-  * volume within 1e-6 of the FP64 form's (relative; measured 9.2e-7 on the
+  * volume within 1e-6 of the FP64 form's (relative; measured 2.6e-7 on the
     watershed, whose open edges let the two forms drain at slightly
     different times);
   * mean |dh| <= 1e-5 m;
   * max |dh| over every interior cell <= 1e-3 m, or, where the yardstick
     already exceeds 5e-4 m, <= 6x the yardstick: FP32 arithmetic (rounding
     of the face fluxes at ~1e-7 of their terms) perturbs more than storage
-    does, up to 5.2x the yardstick's max (the watershed) and 28x its mean;
+    does, up to 5.7x the yardstick's max (the watershed) and 32x its mean;
   * wet/dry mismatches at most 3x the yardstick's (plus 2)."""
 import numpy as np
 import pytest

@@ -21,8 +21,9 @@
 //   * bed tests z1+h1<z2 become h1<dz on the stored face difference;
 //   * the semi-implicit friction uses q = 2m/(1+sqrt(1+4tt)), the same value
 //     as -0.5(m - m sqrt(1+4tt))/tt without its cancellation for small tt;
-//   * the updated depth (flux update and rain) is carried as two floats and
-//     rounded to FP32 stochastically, which keeps mass (sround).
+//   * the updated depth (flux update, rain and point sources) is carried as
+//     two floats and rounded to FP32 stochastically, which keeps mass
+//     (sround).
 #include <hip/hip_runtime.h>
 
 #include "batch.hpp"
@@ -367,6 +368,14 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
     const int ru = r - 1;
     float nh = ph, nqx = pqx, nqy = pqy;
     const bool ru_int = ru >= 1 && ru <= R - 2;
+    // point-source depth of this lane's cell in row ru (src/kernels.h:50-84;
+    // the FP64 depths the close prepared, summed in list order)
+    double sadd = 0.0;
+    if (nsrc > 0 && ru_int) {
+      while (ks < nsrc && src_row[ks] < ru) ks++;
+      for (int k = ks; k < nsrc && src_row[k] == ru; k++)
+        if (src_col[k] == c) sadd += src_add[k];
+    }
     if (ru_int && col_int) {
       // depth ph - dtdx*S carried as hs + he, he the rounding error
       float hs, he;
@@ -421,6 +430,15 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
         nqx = low ? 0.f : nqx;
         nqy = low ? 0.f : nqy;
       }
+      if (sadd != 0.0) {
+        // point sources join the two-float depth before it is rounded, so
+        // their volume is rounded without bias like the rest of the update
+        const float shi = float(sadd);
+        const float slo = float(sadd - double(shi));
+        float e;
+        hs = two_sum(hs, shi, e);
+        he += e + slo;
+      }
       nh = sround(hs, he, int(dither_col + fmix32(dither_step + unsigned(ru) * kDitherRow)));
     }
     // x face (c, c+1) of row r: back side stays in this lane (rhs*0), the
@@ -449,11 +467,6 @@ __device__ __forceinline__ float step_piece(const InstF32& P, int cur, double dt
     c0y += yln;
     c0x += ylt;
 
-    if (nsrc > 0 && ru_int) {
-      while (ks < nsrc && src_row[ks] < ru) ks++;
-      for (int k = ks; k < nsrc && src_row[k] == ru; k++)
-        if (src_col[k] == c) nh = nh + float(src_add[k]);
-    }
     if (open_bc) {
       if (edge_we) {  // this strip holds the west or east ghost column
         const float rh = from_right(nh), rqx = from_right(nqx), rqy = from_right(nqy);
