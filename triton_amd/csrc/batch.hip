@@ -82,16 +82,20 @@ constexpr int kVolBlocks = 256;
 // 0.1 m deep paraboloid fixture).
 constexpr float kDefaultDryEpsF32 = 1e-12f;
 
-// Waves per SIMD of a batched launch (the step kernel's registers allow 5
+// Waves per SIMD of a batched launch (the step kernel's registers allow 6
 // resident, and 5 measured fastest on one large grid: more waves run in later
 // rounds that no longer share their neighbours' halo rows in L2; an uneven
 // ensemble gains from 10-15, which the autotuner finds) and the band heights a
-// small batch is given (it spreads over about one wave per SIMD: each piece
-// recomputes a 4-row halo, but a 400 x 400 grid steps in 28 us with 8-row
-// waves against 81 us with 48-row ones).
+// small batch is given: it spreads over about kSmallWavesPerSimd waves per
+// SIMD with bands of kFloorRowsPerWave..kMinRowsPerWave rows.  Each piece
+// recomputes a 4-row halo, but a lone wave on a SIMD is latency-bound, so
+// short bands on more waves win: a 400 x 400 grid steps in 16.5 us with
+// 2-row bands against 25.3 us with 8-row bands at one wave per SIMD (81 us
+// with 48-row bands), Thacker's 1024 x 1024 in 28.7 against 42.2 us.
 constexpr int kWavesPerSimd = 5;
-constexpr int kFloorRowsPerWave = 8;
+constexpr int kFloorRowsPerWave = 2;
 constexpr int kMinRowsPerWave = 48;
+constexpr int kSmallWavesPerSimd = 4;
 
 }  // namespace
 }  // namespace tr
@@ -338,9 +342,11 @@ int rebuild_work(tr_batch* b) {
       }
       return w;
     };
-    // a batch too small to fill the device spreads over about one wave per
-    // SIMD, with bands of kFloorRowsPerWave..kMinRowsPerWave rows
-    int64_t lo = std::max<int64_t>(kFloorRowsPerWave, std::min<int64_t>(kMinRowsPerWave, total / b->simd_count));
+    // a batch too small to fill the device spreads over about
+    // kSmallWavesPerSimd waves per SIMD, with bands of
+    // kFloorRowsPerWave..kMinRowsPerWave rows
+    int64_t lo = std::max<int64_t>(
+        kFloorRowsPerWave, std::min<int64_t>(kMinRowsPerWave, total / (int64_t(b->simd_count) * kSmallWavesPerSimd)));
     int64_t hi = lo;
     for (int i : order) hi = std::max<int64_t>(hi, b->m[size_t(i)].inst.rows);
     while (lo < hi) {

@@ -68,6 +68,7 @@ def time_steps(batch, steps: int) -> float:
     """Milliseconds per batched step over `steps` steps (after as many
     untimed ones, which also build any graph)."""
     import torch
+    torch.cuda.init()  # an external stream needs torch's device context
     st = torch.cuda.ExternalStream(batch.stream)
     batch.arm(max_steps=2 * steps)
     batch.enqueue(steps)
@@ -108,6 +109,7 @@ def time_sim_steps(sim, steps: int) -> float:
     """Milliseconds per step of a fused simulation over `steps` steps (after
     as many untimed ones)."""
     import torch
+    torch.cuda.init()  # an external stream needs torch's device context
     st = torch.cuda.ExternalStream(sim.stream)
     sim.arm(max_steps=2 * steps)
     sim.enqueue(steps)
