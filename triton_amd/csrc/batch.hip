@@ -29,14 +29,6 @@ namespace {
     if (rc_) return rc_; \
   } while (0)
 
-// Control blocks of all members in one allocation so the host reads them
-// with a single copy at sync points.
-struct alignas(128) CtrlPair {
-  StepCtrl ctrl[2];
-  int cursor;
-  int pad[31];
-};
-
 struct Member {
   bool used = false;
   bool active = false;   // in the work table
@@ -114,6 +106,7 @@ struct tr_batch {
   Piece* d_pieces = nullptr;
   int* d_members = nullptr;    // [cap] active members, in partition order
   int nmembers = 0;
+  int first_member = 0;        // order[0] of the partition
   int any_bc = 0;              // some member in the partition has external boundaries
   int piece_cap = 0;
   int nwork = 0;               // waves per launch
@@ -378,6 +371,7 @@ int rebuild_work(tr_batch* b) {
   if (!order.empty())
     TR_HIP(hipMemcpy(b->d_members, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
   b->nmembers = int(order.size());
+  b->first_member = order.empty() ? 0 : order[0];
   b->nwork = nw;
   b->band_rows = nw > 0 ? int(band_h) : 0;
   b->npieces = int(pieces.size());
@@ -656,7 +650,7 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   cp.cursor = 1;
   RC(put_ctrl(b, id));
   RC(put_inst(b, id));
-  RC(launch_prime_f32(b->d_inst, id, P, s));
+  RC(launch_prime_f32(b->d_inst, b->d_ctrl, id, P, s));
   TR_HIP(hipStreamSynchronize(s));
   b->bytes += mm.bytes;
   mm.active = true;
@@ -714,6 +708,7 @@ namespace {
 bool same_args(const BatchArgs& x, const BatchArgs& y) {
   return x.inst == y.inst && x.pieces == y.pieces &&
          x.nwaves == y.nwaves && x.members == y.members && x.nmembers == y.nmembers &&
+         x.ctrls == y.ctrls && x.first_member == y.first_member &&
          x.any_bc == y.any_bc;
 }
 
@@ -757,7 +752,7 @@ int tr_batch_enqueue(tr_batch* b, int64_t nsteps) {
     return TR_ERR_INVALID;
   }
   if (b->table_dirty) RC(rebuild_work(b));
-  BatchArgs a{b->d_inst, b->d_pieces, b->nwork, b->d_members, b->nmembers, b->any_bc};
+  BatchArgs a{b->d_inst, b->d_ctrl, b->d_pieces, b->nwork, b->d_members, b->nmembers, b->first_member, b->any_bc};
   int64_t k = 0;
   if (b->graph_steps > 0 && a.nwaves > 0 && nsteps >= b->graph_steps) {
     if (!b->gexec || b->gsteps_built != b->graph_steps || !same_args(a, b->gargs)) RC(build_graph(b, a));

@@ -56,6 +56,16 @@ struct InstF32 {
   ExtBC bc;                 // external boundary segments (bc.n == 0: none)
 };
 
+// Control blocks of one member (two slots and the cursor naming the current
+// one); all members' pairs sit in one array, so the host reads every status
+// with one copy and kernels address a member's pair by its index without
+// first reading its descriptor.
+struct alignas(128) CtrlPair {
+  StepCtrl ctrl[2];
+  int cursor;
+  int pad[31];
+};
+
 // A piece is a run of rows of one column strip of one member, and the work
 // of one wave.
 struct Piece {
@@ -67,15 +77,18 @@ struct Piece {
 
 struct BatchArgs {
   const InstF32* inst;
+  CtrlPair* ctrls;           // [member] control blocks (the same memory inst[m].ctrl points into)
   const Piece* pieces;       // [nwaves]: wave w steps pieces[w]
   int nwaves;
   const int* members;        // members in the partition (the close kernel's list)
   int nmembers;
+  int first_member;          // the member when nmembers == 1 (the close then reads no list)
   int any_bc;                // some member has external boundary segments
 };
 
 int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s);
-int launch_prime_f32(const InstF32* d_inst, int member, const InstF32& host_inst, hipStream_t s);
+int launch_prime_f32(const InstF32* d_inst, CtrlPair* d_ctrls, int member, const InstF32& host_inst,
+                     hipStream_t s);
 int fused_grid_x_f32(int cols);
 int default_strip_rows(int rows, int cols);
 

@@ -871,11 +871,11 @@ constexpr int kNoAllocMaxZones = 64;
 constexpr size_t kScratchBytes = 16384;
 // byte offsets into the scratch array
 constexpr size_t kOffCtrl = 0;                                      // StepCtrl[2]
-constexpr size_t kOffZoneAdd = 4096;                                // double[2][64]
+constexpr size_t kOffZoneAdd = (2 * sizeof(StepCtrl) + 255) / 256 * 256;  // double[2][64]
 constexpr size_t kOffZoneCount = kOffZoneAdd + 2 * kNoAllocMaxZones * 8;  // u64[64]
 constexpr size_t kOffZoneArea = kOffZoneCount + kNoAllocMaxZones * 8;     // double[64]
 constexpr size_t kOffFlag = kOffZoneArea + kNoAllocMaxZones * 8;          // int
-constexpr size_t kOffVolPart = 8192;                                // double[kVolBlocks]
+constexpr size_t kOffVolPart = (kOffFlag + 4 + 255) / 256 * 256;          // double[kVolBlocks]
 constexpr size_t kOffVolSum = kOffVolPart + kVolBlocks * 8;          // double
 static_assert(2 * sizeof(StepCtrl) <= kOffZoneAdd, "control blocks overlap the zone table");
 static_assert(kOffFlag + 4 <= kOffVolPart, "zone tables overlap the volume partials");

@@ -532,9 +532,10 @@ __device__ __forceinline__ void publish_min_f32(StepCtrl* cc, float fast, float 
 }
 
 // Batched step: wave w (placed XCD-contiguously) steps piece w and publishes
-// its dt min.  The cursor and both control slots are read together, so the
-// rows' loads wait on three dependent reads (piece, member descriptor,
-// cursor) -- what a small grid's step time is made of.
+// its dt min.  The member's cursor and both control slots are read from the
+// control array by member index, in parallel with its descriptor, so the
+// rows' loads wait on two dependent reads (piece, then descriptor and
+// control) -- what a small grid's step time is made of.
 // 5 waves per SIMD: the step fits in 96 VGPRs without spills.
 // kBC: some member has external boundary segments (a variant of its own, so
 // the common one keeps its registers).
@@ -544,10 +545,11 @@ __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const int lane = threadIdx.x;
   const Piece pc = a.pieces[w];
   const InstF32& P = a.inst[pc.member];
+  CtrlPair& cp = a.ctrls[pc.member];
   // wave-uniform: keeps the member's array bases in scalar registers
-  const int cur = __builtin_amdgcn_readfirstlane(*P.cursor);
-  const StepCtrl& c0 = P.ctrl[0];
-  const StepCtrl& c1 = P.ctrl[1];
+  const int cur = __builtin_amdgcn_readfirstlane(cp.cursor);
+  const StepCtrl& c0 = cp.ctrl[0];
+  const StepCtrl& c1 = cp.ctrl[1];
   const int halt0 = c0.halt, halt1 = c1.halt;
   const double dt0 = c0.dt, dt1 = c1.dt, t0 = c0.simtime, t1 = c1.simtime;
   const long long n0 = c0.step, n1 = c1.step;
@@ -555,7 +557,7 @@ __global__ __launch_bounds__(64, 5) void k_step_f32(BatchArgs a) {
   const unsigned dither_step = unsigned(P.it_count0 + (cur ? n1 : n0)) * kDitherStep;
   const float fast = step_piece<kBC>(P, cur, cur ? dt1 : dt0, cur ? t1 : t0, dither_step, pc.bx, pc.row_first,
                                      pc.row_last, lane);
-  publish_min_f32(P.ctrl + cur, fast, P.cn_dx_f, lane);
+  publish_min_f32(cp.ctrl + cur, fast, P.cn_dx_f, lane);
 }
 
 // Initial dt of one member over every cell (ghosts included).
@@ -572,15 +574,19 @@ __global__ __launch_bounds__(64) void k_prime_f32(const InstF32* inst, int membe
 // One thread per member (members[i], or first + i when members is NULL):
 // prepare the other control slot and flip the member's cursor.  Halted
 // members keep their cursor, so launches queued past a halt are no-ops.
-__global__ __launch_bounds__(64) void k_close_f32(const InstF32* inst, const int* members, int first,
-                                                  int n, int after_step) {
+// The control pair is addressed by member index, so its reads do not wait
+// for the descriptor's.
+__global__ __launch_bounds__(64) void k_close_f32(const InstF32* inst, CtrlPair* ctrls, const int* members,
+                                                  int first, int n, int after_step) {
   const int i = int(blockIdx.x) * kLanes + int(threadIdx.x);
   if (i >= n) return;
-  const InstF32& P = inst[members ? members[i] : first + i];
-  const int cur = *P.cursor;
-  if (P.ctrl[cur].halt) return;
-  close_one(P, cur, after_step != 0);
-  *P.cursor = cur ^ 1;
+  const int m = members ? members[i] : first + i;
+  CtrlPair& cp = ctrls[m];
+  const int cur = cp.cursor;
+  const StepCtrl& cc = cp.ctrl[cur];
+  if (cc.halt) return;
+  prepare_next(inst[m], cc, cp.ctrl[cur ^ 1], dtmin_of(cc), after_step != 0, cur ^ 1);
+  cp.cursor = cur ^ 1;
 }
 
 }  // namespace
@@ -602,16 +608,19 @@ int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s) {
   if (nwaves <= 0) return TR_OK;
   if (a.any_bc) hipLaunchKernelGGL(k_step_f32<true>, dim3(unsigned(nwaves)), dim3(kLanes), 0, s, a);
   else hipLaunchKernelGGL(k_step_f32<false>, dim3(unsigned(nwaves)), dim3(kLanes), 0, s, a);
+  // one member: the close takes its index directly instead of reading the list
+  const bool one = a.nmembers == 1;
   hipLaunchKernelGGL(k_close_f32, dim3(unsigned((a.nmembers + kLanes - 1) / kLanes)), dim3(kLanes), 0, s,
-                     a.inst, a.members, 0, a.nmembers, 1);
+                     a.inst, a.ctrls, one ? nullptr : a.members, one ? a.first_member : 0, a.nmembers, 1);
   return launch_check("step_f32");
 }
 
-int launch_prime_f32(const InstF32* d_inst, int member, const InstF32& host_inst, hipStream_t s) {
+int launch_prime_f32(const InstF32* d_inst, CtrlPair* d_ctrls, int member, const InstF32& host_inst,
+                     hipStream_t s) {
   int64_t n = int64_t(host_inst.rows) * host_inst.cols;
   unsigned g = unsigned(std::min<int64_t>((n + kLanes - 1) / kLanes, 2048));
   hipLaunchKernelGGL(k_prime_f32, dim3(g), dim3(kLanes), 0, s, d_inst, member);
-  hipLaunchKernelGGL(k_close_f32, dim3(1), dim3(kLanes), 0, s, d_inst, nullptr, member, 1, 0);
+  hipLaunchKernelGGL(k_close_f32, dim3(1), dim3(kLanes), 0, s, d_inst, d_ctrls, nullptr, member, 1, 0);
   return launch_check("prime_f32");
 }
 
