@@ -39,6 +39,33 @@ __device__ __forceinline__ float tmin<float>(float a, float b) {
   return fminf(a, b);
 }
 
+// a / b for many a and one b, with the bits of the compiler's IEEE FP64
+// division on this GPU.  That division scales its operands (v_div_scale),
+// refines a hardware reciprocal of b with two Newton steps, forms the
+// quotient with one correction step (v_div_fmas) and fixes up special cases
+// (v_div_fixup).  When a is +0 or a normal number of moderate size and b is a
+// normal number of moderate size -- 2^-900 < |a|, 2^-100 < b < 2^100,
+// |a/b| < 2^900 -- neither operand is scaled, the correction is a plain fma
+// and the fix-up returns its input, so the refined reciprocal can be formed
+// once for b (FixedDiv64) and each quotient costs three instructions.
+// Callers guarantee that range (div_fixed_ok).
+struct FixedDiv64 {
+  double b, y;  // divisor and its refined reciprocal
+};
+__device__ __forceinline__ FixedDiv64 fixed_div64(double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double e0 = fma(-b, r0, 1.0);
+  const double r1 = fma(r0, e0, r0);
+  const double e1 = fma(-b, r1, 1.0);
+  return FixedDiv64{b, fma(r1, e1, r1)};
+}
+__device__ __forceinline__ double div_fixed(double a, const FixedDiv64& d) {
+  const double m = a * d.y;
+  const double f = fma(-d.b, m, a);
+  return fma(f, d.y, m);
+}
+__host__ __device__ inline bool div_fixed_ok(double b) { return b > 0x1p-100 && b < 0x1p100; }
+
 template <class T>
 struct FaceT {
   T d1[3];  // fluctuation sent to the west (x) / south (y) cell

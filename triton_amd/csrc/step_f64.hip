@@ -55,17 +55,22 @@ __global__ __launch_bounds__(64) void k_close_f64(StepParams<double> p, int afte
 // point sources and the discharges to nearest.  FP64 arithmetic on
 // FP32-stored state: the accuracy yardstick of the FP32 form
 // (TR_FORM_F64_R32).
-template <bool kOpen, bool kBC, bool kR32>
-__global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
-  const int lane = threadIdx.x;
-  const StepCtrl* cc = p.ctrl + p.cur;
-  if (cc->halt) return;
+// kFast: the per-face dt/dx scaling and each cell's h dx 0.5 / dt divide by
+// dx and dt, fixed for the launch, through div_fixed (the same bits as the
+// IEEE division when dx and dt are in its range, which the kernel checks once
+// per wave); otherwise every quotient is an IEEE division.
+template <bool kOpen, bool kBC, bool kR32, bool kFast>
+__device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, const StepCtrl* cc, int lane) {
   const double dt = cc->dt;
   const double simtime = cc->simtime;
   const int roff_row = cc->roff_row;
   const double* __restrict__ src_add = p.src_add + p.cur * p.nsrc;
   const unsigned dither_step = unsigned(p.it_count0 + cc->step) * kDitherStep;
   const double dx = p.dx, hextra = p.hextra;
+  // dt * d / dx of a face fluctuation d, which the flush leaves +0 or at
+  // least 1e-12 in magnitude (src/kernels_flux.h:255-273)
+  const FixedDiv64 rdx = fixed_div64(dx), rdt = fixed_div64(dt);
+  auto scale = [&](double d) { return kFast ? div_fixed(dt * d, rdx) : dt * d / dx; };
   const int R = p.rows, C = p.cols;
   const int tile = xcd_slot(int(blockIdx.x), int(p.nblocks));
   const int bx = tile % p.nbx, by = tile / p.nbx;
@@ -117,7 +122,22 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
       }
     }
     const double ssq = fmax(sqrt(sh), 0.0);
-    const Cell64 sk = cell64(sh, sqx, sqy, dx, dt, hextra);
+    Cell64 sk;
+    if (kFast) {
+      // h dx 0.5 / dt by div_fixed unless some lane's numerator is below its
+      // range (then the whole wave divides)
+      const double num = sh * dx * 0.5;
+      sk.u = 0.0;
+      sk.v = 0.0;
+      if (sh >= hextra) {
+        sk.u = sqx / sh;
+        sk.v = sqy / sh;
+      }
+      if (__any(num > 0.0 && num < 0x1p-900)) sk.hd = num / dt;
+      else sk.hd = div_fixed(num, rdt);
+    } else {
+      sk = cell64(sh, sqx, sqy, dx, dt, hextra);
+    }
 
     // B: x face (c, c+1) of row r
     double x0h = 0.0, x0x = 0.0, x0y = 0.0, x1h, x1x, x1y;
@@ -129,12 +149,12 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
       if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
         Face64 f;
         if (roe_face_f64<1, 0>(sh, hR, sqx, qxR, sqy, qyR, sk, kR, sz, zR, ssq, sqR, f)) {
-          x0h = dt * f.d1[0] / dx;
-          x0x = dt * f.d1[1] / dx;
-          x0y = dt * f.d1[2] / dx;
-          e0 = dt * f.d2[0] / dx;
-          e1 = dt * f.d2[1] / dx;
-          e2 = dt * f.d2[2] / dx;
+          x0h = scale(f.d1[0]);
+          x0x = scale(f.d1[1]);
+          x0y = scale(f.d1[2]);
+          e0 = scale(f.d2[0]);
+          e1 = scale(f.d2[1]);
+          e2 = scale(f.d2[2]);
         }
       }
       x1h = from_left(e0);
@@ -147,12 +167,12 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
     if (row_ok && r >= 1 && col_int) {
       Face64 f;
       if (roe_face_f64<0, 1>(sh, ph, sqx, pqx, sqy, pqy, sk, pk, sz, pz, ssq, psq, f)) {
-        y0h = dt * f.d1[0] / dx;
-        y0x = dt * f.d1[1] / dx;
-        y0y = dt * f.d1[2] / dx;
-        y1h = dt * f.d2[0] / dx;
-        y1x = dt * f.d2[1] / dx;
-        y1y = dt * f.d2[2] / dx;
+        y0h = scale(f.d1[0]);
+        y0x = scale(f.d1[1]);
+        y0y = scale(f.d1[2]);
+        y1h = scale(f.d2[0]);
+        y1x = scale(f.d2[1]);
+        y1y = scale(f.d2[2]);
       }
     }
 
@@ -238,6 +258,16 @@ __global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
   }
   best = wave_min_bits(best);
   if (lane == 0) publish_min(p.ctrl + p.cur, best);
+}
+
+template <bool kOpen, bool kBC, bool kR32>
+__global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
+  const int lane = threadIdx.x;
+  const StepCtrl* cc = p.ctrl + p.cur;
+  if (cc->halt) return;
+  // dt * 1e-12 must stay above 2^-900 for the scaled fluctuations
+  if (div_fixed_ok(p.dx) && div_fixed_ok(cc->dt)) step_strip_f64<kOpen, kBC, kR32, true>(p, cc, lane);
+  else step_strip_f64<kOpen, kBC, kR32, false>(p, cc, lane);
 }
 
 // Host launchers (called by the simulation driver).
