@@ -160,18 +160,33 @@ __device__ __forceinline__ bool roe_face_t(T h1, T h2, T qx1, T qx2, T qy1, T qy
 
   hp = h1 + a0;
   if (l0 * l2 < 0.0 && hp > 0.0 && h1 > 0.0 && h2 > 0.0) {
-    b0 = tmax<T>(b0, l0 / (l0 + e2) * (a0 * l0 - c1.hd + e2 * h2));
-    b0 = tmin<T>(b0, l2 / (l2 + e0) * (-a2 * l2 + c2.hd + e0 * h1));
+    // l0 / (l0 + e2) is exactly 1 when e2 == 0 (no entropy fix on wave 2):
+    // here l0 != 0 (l0 l2 < 0), so only a lane with e2 != 0 or l0 not finite
+    // needs the quotient, and a wave without one skips both divisions
+    T f0 = T(1), f2 = T(1);
+    if (__any(e0 != 0.0 || e2 != 0.0 || !isfinite(l0) || !isfinite(l2))) {
+      f0 = l0 / (l0 + e2);
+      f2 = l2 / (l2 + e0);
+    }
+    b0 = tmax<T>(b0, f0 * (a0 * l0 - c1.hd + e2 * h2));
+    b0 = tmin<T>(b0, f2 * (-a2 * l2 + c2.hd + e0 * h1));
   }
   const T b2 = -b0;
 
   const T one = T(1), zero = T(0);
   T d10 = zero, d11 = zero, d12 = zero, d20 = zero, d21 = zero, d22 = zero;
-  const T sL = hp - b0 / l0;
-  const T sR = hp + b2 / l2;
-  if (sR < -eps && h2 < eps) {
+  // the dry-side tests hold only where a side is below eps, so a wave with
+  // no such side skips their two divisions
+  bool to_left = false, to_right = false;
+  if (__any(h1 < eps || h2 < eps)) {
+    const T sL = hp - b0 / l0;
+    const T sR = hp + b2 / l2;
+    to_left = sR < -eps && h2 < eps;
+    to_right = sL < -eps && h1 < eps;
+  }
+  if (to_left) {
     d10 += (l0 * a0 - b0) + (l2 * a2 - b2);
-  } else if (sL < -eps && h1 < eps) {
+  } else if (to_right) {
     d20 += (l0 * a0 - b0) + (l2 * a2 - b2);
   } else {
     // wave 0: eigenvector (1, ev01, ev02)
