@@ -79,6 +79,29 @@ def test_fused_equals_oracle_shapes(gpu, shape, open_bc):
     assert np.array_equal(st["maxh"], orc.a["maxh"])
 
 
+@pytest.mark.parametrize("variant", ["tiny_depths", "dx_below_range", "dx_above_range"])
+def test_fused_quotient_ranges(gpu, variant):
+    """The fused FP64 step divides by dx and dt through a reciprocal refined
+    once per launch, which has the IEEE division's bits only inside a range;
+    outside it the step divides.  Cells with depths near 1e-300 m (the
+    waves holding them divide h dx/2dt), and grids whose dx lies
+    below 2^-100 or above 2^100 (the whole launch falls back), still equal
+    the oracle bit for bit, signed zeros included."""
+    if variant == "tiny_depths":
+        case = synthetic(37, 83, seed=91, open_bc=1, friction=False, dur=6.0)
+        h = case.arrays["h"]
+        h[5, 7], h[20, 40], h[30, 70] = 1e-300, 3e-295, 1e-290
+    else:
+        # dx of 2^-101 m gives dt near 1e-31 s, dx of 2^101 m dt near 4e29 s
+        case = synthetic(37, 83, seed=91, open_bc=1, friction=False, runoff=False, sources=False,
+                         dur=1e60)
+        case.scalars["dx"] = 2.0 ** (-101 if variant == "dx_below_range" else 101)
+    orc, ro, rs, st = run_both(case, ta.FORM_F64_FUSED, max_steps=60)
+    assert rs["steps"] == ro["steps"] >= 40 and rs["simtime"] == ro["simtime"]
+    for k in ("h", "qx", "qy"):
+        assert np.array_equal(st[k].view(np.uint64), orc.state[k].view(np.uint64)), k
+
+
 @pytest.mark.parametrize("form", [ta.FORM_F64_UNFUSED, ta.FORM_F64_FUSED])
 def test_friction_forcing_open(gpu, form):
     case = synthetic(150, 170, seed=11, open_bc=1, friction=True, dur=40.0)
