@@ -68,7 +68,9 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
   const unsigned dither_step = unsigned(p.it_count0 + cc->step) * kDitherStep;
   const double dx = p.dx, hextra = p.hextra;
   // dt * d / dx of a face fluctuation d, which the flush leaves +0 or at
-  // least 1e-12 in magnitude (src/kernels_flux.h:255-273)
+  // least 1e-12 in magnitude (src/kernels_flux.h:255-273); a fluctuation
+  // that is already infinite (a run that has overflowed) gives NaN here where
+  // the IEEE division gives an infinity
   const FixedDiv64 rdx = fixed_div64(dx), rdt = fixed_div64(dt);
   auto scale = [&](double d) { return kFast ? div_fixed(dt * d, rdx) : dt * d / dx; };
   const int R = p.rows, C = p.cols;
@@ -124,8 +126,9 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
     const double ssq = fmax(sqrt(sh), 0.0);
     Cell64 sk;
     if (kFast) {
-      // h dx 0.5 / dt by div_fixed unless some lane's numerator is below its
-      // range (then the whole wave divides)
+      // h dx 0.5 / dt by div_fixed unless some lane's numerator is outside
+      // its range (below 2^-900, above 2^900, not finite: then the whole wave
+      // divides)
       const double num = sh * dx * 0.5;
       sk.u = 0.0;
       sk.v = 0.0;
@@ -133,7 +136,7 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
         sk.u = sqx / sh;
         sk.v = sqy / sh;
       }
-      if (__any(num > 0.0 && num < 0x1p-900)) sk.hd = num / dt;
+      if (__any(!(num == 0.0 || (num >= 0x1p-900 && num <= 0x1p900)))) sk.hd = num / dt;
       else sk.hd = div_fixed(num, rdt);
     } else {
       sk = cell64(sh, sqx, sqy, dx, dt, hextra);
