@@ -96,6 +96,43 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
   const float aL2 = L.un + kSqrtGf * L.sq, aR2 = R.un + kSqrtGf * R.sq;
   const bool fix0 = aL0 < 0.f && aR0 > 0.f;
   const bool fix2 = aL2 < 0.f && aR2 > 0.f;
+  {
+    // The face a wave sees almost everywhere in a rained-on catchment (on
+    // the ensemble at peak rain, 99.8-100 % of the wave-rows of the members
+    // sampled): every lane's face has both sides wet, is subcritical
+    // (l0 < 0 < l2) with hp > 0, so the wet-wet clip applies, and needs no
+    // entropy fix.  Such a wave takes the general path's operations without
+    // its tests, selects and branches.  Which path a wave takes depends only
+    // on the cells of its row window, so a member's bits still do not depend
+    // on the batch it runs in.
+    const float dh = R.h - L.h;
+    const float w = ((R.qn - L.qn) - ub * dh) * ic;
+    const float a0 = 0.5f * (dh - w), a2 = 0.5f * (dh + w);
+    const float hp = L.h + a0;
+    const bool common = !fix0 && !fix2 && l0 < 0.f && l2 > 0.f && hp > 0.f && L.h >= eps && R.h >= eps;
+    if (__all(common)) {
+      const float t1 = ub * ((R.qt - L.qt) - vb * dh);
+      const bool up = dz >= 0.f;
+      const float hp0 = up ? L.h : R.h;
+      const float dzp = hp0 < fabsf(dz) ? (up ? hp0 : -hp0) : dz;
+      const float b0 = (0.5f * kGf) * (hp0 - 0.5f * fabsf(dzp)) * dzp * ic;
+      const float W0 = l0 * a0 - b0, W2 = l2 * a2 + b0;
+      const float Ld = L.h * hdxdt, Rd = R.h * hdxdt;
+      const float S = W0 + W2;
+      const float w0 = fmaxf(fminf(W0, Ld), S - Rd);
+      const float w2 = fminf(fmaxf(W2, S - Ld), Rd);
+      // the back side takes wave 0 (l0 < 0), the front side wave 2
+      const float bm = w0, bn = w0 * l0, bt = w0 * vb + (ub > 0.f ? 0.f : t1);
+      const float tm = w0 + w2, tn = w0 * l0 + w2 * l2, tt = tm * vb + t1;
+      lm = bm;
+      ln = bn;
+      lt = bt;
+      rm = tm - bm;
+      rn = tn - bn;
+      rt = tt - bt;
+      return;
+    }
+  }
   if (__any(fix0 || fix2)) {
     const float sp0 = aL0 * (aR0 - l0) * rcpf(aR0 - aL0);
     const float sp2 = aR2 * (l2 - aL2) * rcpf(aR2 - aL2);
