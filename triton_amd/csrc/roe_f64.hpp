@@ -184,34 +184,64 @@ __device__ __forceinline__ bool roe_face_t(T h1, T h2, T qx1, T qx2, T qy1, T qy
     to_left = sR < -eps && h2 < eps;
     to_right = sL < -eps && h1 < eps;
   }
-  if (to_left) {
-    d10 += (l0 * a0 - b0) + (l2 * a2 - b2);
-  } else if (to_right) {
-    d20 += (l0 * a0 - b0) + (l2 * a2 - b2);
+  // A face with no dry-side redirection, no entropy fix, wave 0 to the back
+  // side and wave 2 to the front (l0 <= 0 < l2), and finite wave strengths
+  // and eigenvectors -- almost every face of a wet catchment -- receives,
+  // besides the three waves, only products with zero (the entropy parts,
+  // lamE[1] and wave 1's weight on the mass).  Those add signed zeros, which
+  // can change only the sign of a zero sum, and the flush below turns every
+  // zero into +0, so a wave whose faces are all of this kind skips them and
+  // the general path's selects: the same flushed values.
+  const bool plain = !to_left && !to_right && !(l0 > 0.0) && l2 > 0.0 && e0 == 0.0 && e2 == 0.0 &&
+                     isfinite(a0) && isfinite(a1) && isfinite(a2) && isfinite(ubar) && isfinite(vbar) &&
+                     isfinite(c);
+  if (__all(plain)) {
+    const T w0 = l0 * a0 - b0;
+    const T w1 = l1 * a1 - zero;
+    const T w2 = l2 * a2 - b2;
+    d10 = w0 * one;
+    d11 = w0 * ev01;
+    d12 = w0 * ev02;
+    d20 = w2 * one;
+    d21 = w2 * ev21;
+    d22 = w2 * ev22;
+    if (l1 > 0.0) {
+      d21 = w1 * ev11 + d21;
+      d22 = w1 * ev12 + d22;
+    } else {
+      d11 = d11 + w1 * ev11;
+      d12 = d12 + w1 * ev12;
+    }
   } else {
-    // wave 0: eigenvector (1, ev01, ev02)
-    T w = l0 * a0 - b0;
-    if (l0 > 0.0) { d20 += w * one; d21 += w * ev01; d22 += w * ev02; }
-    else          { d10 += w * one; d11 += w * ev01; d12 += w * ev02; }
-    // wave 1: eigenvector (0, ev11, ev12), beta1 = 0
-    w = l1 * a1 - zero;
-    if (l1 > 0.0) { d20 += w * zero; d21 += w * ev11; d22 += w * ev12; }
-    else          { d10 += w * zero; d11 += w * ev11; d12 += w * ev12; }
-    // wave 2: eigenvector (1, ev21, ev22)
-    w = l2 * a2 - b2;
-    if (l2 > 0.0) { d20 += w * one; d21 += w * ev21; d22 += w * ev22; }
-    else          { d10 += w * one; d11 += w * ev21; d12 += w * ev22; }
-  }
-  // entropy-fix parts; lamE[1] is always 0
-  {
-    T w = e0 * a0;
-    if (e0 > 0.0) { d20 += w * one; d21 += w * ev01; d22 += w * ev02; }
-    else          { d10 += w * one; d11 += w * ev01; d12 += w * ev02; }
-    w = zero * a1;
-    d10 += w * zero; d11 += w * ev11; d12 += w * ev12;
-    w = e2 * a2;
-    if (e2 > 0.0) { d20 += w * one; d21 += w * ev21; d22 += w * ev22; }
-    else          { d10 += w * one; d11 += w * ev21; d12 += w * ev22; }
+    if (to_left) {
+      d10 += (l0 * a0 - b0) + (l2 * a2 - b2);
+    } else if (to_right) {
+      d20 += (l0 * a0 - b0) + (l2 * a2 - b2);
+    } else {
+      // wave 0: eigenvector (1, ev01, ev02)
+      T w = l0 * a0 - b0;
+      if (l0 > 0.0) { d20 += w * one; d21 += w * ev01; d22 += w * ev02; }
+      else          { d10 += w * one; d11 += w * ev01; d12 += w * ev02; }
+      // wave 1: eigenvector (0, ev11, ev12), beta1 = 0
+      w = l1 * a1 - zero;
+      if (l1 > 0.0) { d20 += w * zero; d21 += w * ev11; d22 += w * ev12; }
+      else          { d10 += w * zero; d11 += w * ev11; d12 += w * ev12; }
+      // wave 2: eigenvector (1, ev21, ev22)
+      w = l2 * a2 - b2;
+      if (l2 > 0.0) { d20 += w * one; d21 += w * ev21; d22 += w * ev22; }
+      else          { d10 += w * one; d11 += w * ev21; d12 += w * ev22; }
+    }
+    // entropy-fix parts; lamE[1] is always 0
+    {
+      T w = e0 * a0;
+      if (e0 > 0.0) { d20 += w * one; d21 += w * ev01; d22 += w * ev02; }
+      else          { d10 += w * one; d11 += w * ev01; d12 += w * ev02; }
+      w = zero * a1;
+      d10 += w * zero; d11 += w * ev11; d12 += w * ev12;
+      w = e2 * a2;
+      if (e2 > 0.0) { d20 += w * one; d21 += w * ev21; d22 += w * ev22; }
+      else          { d10 += w * one; d11 += w * ev21; d12 += w * ev22; }
+    }
   }
   f.d1[0] = fabs(d10) < eps ? zero : d10;
   f.d1[1] = fabs(d11) < eps ? zero : d11;
