@@ -283,8 +283,8 @@ def run_ensemble(args):
 def fp64_plain(members, spinup_s, warmup, steps, keep=None):
     """The same members stepped one at a time by the plain FP64 fused form
     (one launch per step per member) over the same window as the FP32 batch
-    (advanced to spinup_s, then the same number of untimed steps, the strip
-    height autotuned during them), timed on each member's stream.  keep: a list that receives the first members'
+    (advanced to spinup_s, then at least the same number of untimed steps,
+    the strip height and kernel build autotuned during them), timed on each member's stream.  keep: a list that receives the first members'
     spun-up states as cases (the CPU baseline's sample)."""
     import torch
     from triton_amd import scenarios
@@ -300,8 +300,9 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
             if keep is not None and len(keep) < 8:
                 keep.append(scenarios.restart_at(m, sim.state(), st["simtime"]))
         base = int(sim.sync()["steps"])
-        # rows per wave, tuned with real steps that count towards the warmup
-        strips.append(sim.autotune(steps=6, cache=False))
+        # rows per wave and kernel build, tuned with real steps that count
+        # towards the warmup
+        strips.append((sim.autotune(steps=3, cache=False), sim.waves))
         tuned = int(sim.sync()["steps"]) - base
         if warmup > tuned:
             sim.run(max_steps=warmup - tuned)
@@ -326,7 +327,8 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
             "steps_per_member": steps,
             "form": "fp64_fused, members one at a time", "kernel": "k_step_f64",
             "bytes_per_cell": b_alg(False, True, True),
-            "autotuned_strip_rows": sorted(set(strips))}
+            "autotuned_strip_rows": sorted(set(r for r, _ in strips)),
+            "autotuned_waves_per_simd": {str(w): sum(1 for _, x in strips if x == w) for w in (3, 4)}}
 
 
 def serving(args, capacity_gcups, waves_per_simd=0):
@@ -403,6 +405,10 @@ def run_single(args):
         st = sim.run(t_stop=args.spinup_s)
         desc["spinup_s"] = args.spinup_s
         case = scenarios.restart_at(case, sim.state(), st["simtime"])  # the CPU baseline's sample
+    if not f32 and args.workload != "circular":
+        # rows per wave and kernel build of the FP64 step, tuned with real
+        # steps (the dam break is too short for it)
+        desc["autotuned"] = {"strip_rows": sim.autotune(steps=3, cache=False), "waves_per_simd": sim.waves}
     sim.run(max_steps=args.warmup)
     st0 = sim.sync()
     sim.arm(max_steps=args.steps)

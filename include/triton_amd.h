@@ -293,6 +293,12 @@ int tr_sim_set_obs_halt(tr_sim *s, int32_t on);
  * (the autotuner's knob).  tr_sim_strip_rows: the current value. */
 int tr_sim_set_strip_rows(tr_sim *s, int32_t strip_rows);
 int32_t tr_sim_strip_rows(const tr_sim *s);
+/* Fused FP64 forms: the build of the step kernel for the launches queued
+ * next, 3 waves per SIMD (0 selects it; the default) or 4 (fewer registers,
+ * a few values spilled); results do not depend on it (the autotuner's other
+ * knob).  tr_sim_waves: the current value. */
+int tr_sim_set_waves(tr_sim *s, int32_t waves_per_simd);
+int32_t tr_sim_waves(const tr_sim *s);
 /* The hipStream_t the simulation runs on. */
 void *tr_sim_stream(tr_sim *s);
 /* Bytes of device memory held. */

@@ -84,25 +84,33 @@ def test_autotune_changes_speed_only(gpu, tmp_path, monkeypatch):
 
 
 def test_fp64_strip_autotune_changes_speed_only(gpu, tmp_path, monkeypatch):
-    """The FP64 fused step's strip height (rows per wave) is tuned with real
-    steps among the candidates and the size-based default; a run that tunes
-    equals, bit for bit, a run that takes the same steps with the default."""
-    from triton_amd.tune import DEFAULT_STRIPS
+    """The FP64 fused step's strip height (rows per wave) and kernel build
+    (three or four waves per SIMD) are tuned with real steps among the
+    candidates and the size-based default; a run that tunes equals, bit for
+    bit, a run that takes the same steps with the defaults, and so does a run
+    that alternates the two builds step by step."""
+    from triton_amd.tune import DEFAULT_STRIPS, DEFAULT_WAVES
     monkeypatch.setenv("TR_TUNE_CACHE", str(tmp_path / "tune.json"))
     case = members()[0]
     out = []
-    for tune in (False, True):
+    for mode in ("default", "tune", "alternate"):
         with ta.Simulation(case, ta.FORM_F64_FUSED) as sim:
             default = sim.strip_rows
-            n = len(DEFAULT_STRIPS) + (default not in DEFAULT_STRIPS)
-            if tune:
+            n = (len(DEFAULT_STRIPS) + (default not in DEFAULT_STRIPS)) * len(DEFAULT_WAVES)
+            if mode == "tune":
                 best = sim.autotune(steps=3)
                 assert best in DEFAULT_STRIPS or best == default
-                assert sim.strip_rows == best
+                assert sim.strip_rows == best and sim.waves in DEFAULT_WAVES
+            elif mode == "alternate":
+                for k in range(2 * 3 * n):
+                    sim.set_waves(3 + k % 2)
+                    sim.run(max_steps=1)
             else:
+                assert sim.waves == 3
                 sim.run(max_steps=2 * 3 * n)
             st = sim.run(max_steps=20)
             assert st["steps"] == 2 * 3 * n + 20
             out.append(sim.state())
-    for k in ("h", "qx", "qy"):
-        np.testing.assert_array_equal(out[0][k], out[1][k])
+    for other in out[1:]:
+        for k in ("h", "qx", "qy"):
+            np.testing.assert_array_equal(out[0][k], other[k])

@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
         "tr_copy_observation_points_f64": (C.c_int, [i32, vp, vp, vp, vp]),
         "tr_sim_stream": (vp, [vp]),
         "tr_sim_set_strip_rows": (C.c_int, [vp, i32]),
+        "tr_sim_set_waves": (C.c_int, [vp, i32]),
+        "tr_sim_waves": (i32, [vp]),
         "tr_sim_strip_rows": (i32, [vp]),
         "tr_sim_device_bytes": (i64, [vp]),
         "tr_sim_destroy": (None, [vp]),
@@ -504,6 +506,15 @@ class Simulation:
     @property
     def strip_rows(self) -> int:
         return int(lib().tr_sim_strip_rows(self._h))
+
+    def set_waves(self, waves_per_simd: int) -> None:
+        """Fused FP64 forms: the step kernel's build, 3 or 4 waves per SIMD
+        (0: 3).  Results do not change."""
+        check(lib().tr_sim_set_waves(self._h, int(waves_per_simd)))
+
+    @property
+    def waves(self) -> int:
+        return int(lib().tr_sim_waves(self._h))
 
     def autotune(self, candidates=None, steps: int = 8, cache: bool = True) -> int:
         """Fused FP64 forms: time a few strip heights with real steps and keep

@@ -65,6 +65,14 @@ __device__ __forceinline__ double div_fixed(double a, const FixedDiv64& d) {
   return fma(f, d.y, m);
 }
 __host__ __device__ inline bool div_fixed_ok(double b) { return b > 0x1p-100 && b < 0x1p100; }
+// A value every lane holds, made explicitly wave-uniform (read from the first
+// lane) so the compiler keeps it in scalar registers.
+__device__ __forceinline__ double uniform_f64(double x) {
+  const unsigned long long v = __double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(v));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(v >> 32));
+  return __longlong_as_double((long long)((static_cast<unsigned long long>(hi) << 32) | lo));
+}
 
 template <class T>
 struct FaceT {

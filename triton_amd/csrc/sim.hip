@@ -77,6 +77,7 @@ struct tr_sim {
   double t_stop = INFINITY;        // limits armed for queued steps
   int64_t step_stop = INT64_MAX;
   int strip_rows = 64;
+  int waves4 = 0;                  // FP64 fused step: the four-waves-per-SIMD build
   // volume reduction
   double* vpart = nullptr;
   double* vsum = nullptr;
@@ -285,6 +286,7 @@ StepParams<double> fused_params(tr_sim* s, int64_t launch_index) {
   p.rows = v.rows;
   p.cols = v.cols;
   p.strip_rows = s->strip_rows;
+  p.waves4 = s->waves4;
   p.nbx = fused_grid_x(v.cols);
   p.nby = (v.rows - 2 + s->strip_rows - 1) / s->strip_rows;
   p.nblocks = unsigned(p.nbx) * unsigned(p.nby);
@@ -837,6 +839,22 @@ int tr_sim_set_strip_rows(tr_sim* s, int32_t strip_rows) {
 }
 
 int32_t tr_sim_strip_rows(const tr_sim* s) { return s ? s->strip_rows : 0; }
+
+int tr_sim_set_waves(tr_sim* s, int32_t waves_per_simd) {
+  if (!s || !(waves_per_simd == 0 || waves_per_simd == 3 || waves_per_simd == 4)) {
+    set_error("tr_sim_set_waves: waves_per_simd must be 0, 3 or 4");
+    return TR_ERR_INVALID;
+  }
+  if (s->form != TR_FORM_F64_FUSED && s->form != TR_FORM_F64_R32) {
+    set_error("tr_sim_set_waves: only the fused FP64 forms have this build choice");
+    return TR_ERR_STATE;
+  }
+  TR_HIP(hipStreamSynchronize(s->stream));
+  s->waves4 = waves_per_simd == 4 ? 1 : 0;
+  return TR_OK;
+}
+
+int32_t tr_sim_waves(const tr_sim* s) { return s ? (s->waves4 ? 4 : 3) : 0; }
 
 void* tr_sim_stream(tr_sim* s) {
   if (!s) return nullptr;

@@ -98,6 +98,7 @@ struct StepParams {
   long long it_count0;      // iterations before the start (hotstart)
   ExtBC bc;                 // external boundary segments (bc.n == 0: none)
   int round32;              // FP64 form: round updated h, qx, qy to FP32 (TR_FORM_F64_R32)
+  int waves4;               // FP64 fused step: the build for four waves per SIMD (else three)
 };
 
 }  // namespace tr

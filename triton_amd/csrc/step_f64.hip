@@ -71,7 +71,10 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
   // least 1e-12 in magnitude (src/kernels_flux.h:255-273); a fluctuation
   // that is already infinite (a run that has overflowed) gives NaN here where
   // the IEEE division gives an infinity
-  const FixedDiv64 rdx = fixed_div64(dx), rdt = fixed_div64(dt);
+  FixedDiv64 rdx = fixed_div64(dx), rdt = fixed_div64(dt);
+  // wave-uniform: the reciprocals stay in scalar registers for the whole strip
+  rdx.y = uniform_f64(rdx.y);
+  rdt.y = uniform_f64(rdt.y);
   auto scale = [&](double d) { return kFast ? div_fixed(dt * d, rdx) : dt * d / dx; };
   const int R = p.rows, C = p.cols;
   const int tile = xcd_slot(int(blockIdx.x), int(p.nblocks));
@@ -96,10 +99,11 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
   double ph = 0.0, pqx = 0.0, pqy = 0.0, pz = 0.0, psq = 0.0, pn2 = 0.0;
   Cell64 pk{0.0, 0.0, 0.0};
   double prate = 0.0;
-  // face contributions of the previous row: east side of the x face (rhs*0),
-  // west side (rhs*1), north y face (rhs*0)
-  double px0h = 0.0, px0x = 0.0, px0y = 0.0, px1h = 0.0, px1x = 0.0, px1y = 0.0;
-  double py0h = 0.0, py0x = 0.0, py0y = 0.0;
+  // the west side of the previous row's x faces (rhs*1)
+  double px1h = 0.0, px1x = 0.0, px1y = 0.0;
+  // the previous row's two back-side sums, x face then y face (the
+  // reference's rhs*0 = flux_x's value + flux_y's), summed when both are known
+  double ps0h = 0.0, ps0x = 0.0, ps0y = 0.0;
   // updated rows r-2 (A) and r-3 (B)
   double ah = 0.0, aqx = 0.0, aqy = 0.0, az = 0.0;
   double bh = 0.0, bz = 0.0;
@@ -142,29 +146,6 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
       sk = cell64(sh, sqx, sqy, dx, dt, hextra);
     }
 
-    // B: x face (c, c+1) of row r
-    double x0h = 0.0, x0x = 0.0, x0y = 0.0, x1h, x1x, x1y;
-    {
-      const double hR = from_right(sh), qxR = from_right(sqx), qyR = from_right(sqy);
-      const double zR = from_right(sz), sqR = from_right(ssq);
-      const Cell64 kR{from_right(sk.u), from_right(sk.v), from_right(sk.hd)};
-      double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-      if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
-        Face64 f;
-        if (roe_face_f64<1, 0>(sh, hR, sqx, qxR, sqy, qyR, sk, kR, sz, zR, ssq, sqR, f)) {
-          x0h = scale(f.d1[0]);
-          x0x = scale(f.d1[1]);
-          x0y = scale(f.d1[2]);
-          e0 = scale(f.d2[0]);
-          e1 = scale(f.d2[1]);
-          e2 = scale(f.d2[2]);
-        }
-      }
-      x1h = from_left(e0);
-      x1x = from_left(e1);
-      x1y = from_left(e2);
-    }
-
     // C: y face between row r (south) and row r-1 (north)
     double y0h = 0.0, y0x = 0.0, y0y = 0.0, y1h = 0.0, y1x = 0.0, y1y = 0.0;
     if (row_ok && r >= 1 && col_int) {
@@ -184,8 +165,8 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
     double nh = ph, nqx = pqx, nqy = pqy;
     const bool ru_int = ru >= 1 && ru <= R - 2;
     if (ru_int && col_int) {
-      update_cell_f64(ph, pqx, pqy, pn2, px0h + py0h, px1h + y1h, px0x + py0x, px1x + y1x,
-                      px0y + py0y, px1y + y1y, dt, hextra, nh, nqx, nqy);
+      update_cell_f64(ph, pqx, pqy, pn2, ps0h, px1h + y1h, ps0x, px1x + y1x, ps0y, px1y + y1y, dt, hextra,
+                      nh, nqx, nqy);
       if (p.num_runoffs > 0) {
         double hc = nh;
         hc += prate * dt;
@@ -252,19 +233,47 @@ __device__ __forceinline__ void step_strip_f64(const StepParams<double>& p, cons
       }
     }
 
+    // B: x face (c, c+1) of row r, after the row above is finished, so its
+    // face sums are no longer live beside this face's temporaries
+    double x0h = 0.0, x0x = 0.0, x0y = 0.0, x1h, x1x, x1y;
+    {
+      const double hR = from_right(sh), qxR = from_right(sqx), qyR = from_right(sqy);
+      const double zR = from_right(sz), sqR = from_right(ssq);
+      const Cell64 kR{from_right(sk.u), from_right(sk.v), from_right(sk.hd)};
+      double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+      if (row_ok && r >= 1 && r <= R - 2 && xface_ok) {
+        Face64 f;
+        if (roe_face_f64<1, 0>(sh, hR, sqx, qxR, sqy, qyR, sk, kR, sz, zR, ssq, sqR, f)) {
+          x0h = scale(f.d1[0]);
+          x0x = scale(f.d1[1]);
+          x0y = scale(f.d1[2]);
+          e0 = scale(f.d2[0]);
+          e1 = scale(f.d2[1]);
+          e2 = scale(f.d2[2]);
+        }
+      }
+      x1h = from_left(e0);
+      x1x = from_left(e1);
+      x1y = from_left(e2);
+    }
+
     // rotate the window
     bh = ah; bz = az;
     ah = nh; aqx = nqx; aqy = nqy; az = pz;
     ph = sh; pqx = sqx; pqy = sqy; pz = sz; psq = ssq; pn2 = sn2; prate = srate; pk = sk;
-    px0h = x0h; px0x = x0x; px0y = x0y; px1h = x1h; px1x = x1x; px1y = x1y;
-    py0h = y0h; py0x = y0x; py0y = y0y;
+    ps0h = x0h + y0h; ps0x = x0x + y0x; ps0y = x0y + y0y;
+    px1h = x1h; px1x = x1x; px1y = x1y;
   }
   best = wave_min_bits(best);
   if (lane == 0) publish_min(p.ctrl + p.cur, best);
 }
 
-template <bool kOpen, bool kBC, bool kR32>
-__global__ __launch_bounds__(64) void k_step_f64(StepParams<double> p) {
+// kW4: built for four waves per SIMD (at most 128 VGPRs, which spills a few
+// values the compiler picks) instead of three; the same operations, so the
+// same bits.  Which is faster depends on the grid and the strip height, so it
+// is a launch knob the autotuner sets (tr_sim_set_waves).
+template <bool kOpen, bool kBC, bool kR32, bool kW4>
+__global__ __launch_bounds__(64, kW4 ? 4 : 1) void k_step_f64(StepParams<double> p) {
   const int lane = threadIdx.x;
   const StepCtrl* cc = p.ctrl + p.cur;
   if (cc->halt) return;
@@ -282,21 +291,36 @@ int launch_prime_f64(const StepParams<double>& p, hipStream_t s) {
   return launch_check("prime_f64");
 }
 
-int launch_step_f64(const StepParams<double>& p, hipStream_t s) {
-  dim3 grid(p.nblocks);
-  const bool bc = p.bc.n > 0;
+template <bool kW4>
+void launch_step_f64_w(const StepParams<double>& p, hipStream_t s) {
+  const dim3 grid(p.nblocks), blk(kLanes);
   if (p.round32) {
-    if (p.open_bc && bc) hipLaunchKernelGGL((k_step_f64<true, true, true>), grid, dim3(kLanes), 0, s, p);
-    else if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false, true>), grid, dim3(kLanes), 0, s, p);
-    else if (bc) hipLaunchKernelGGL((k_step_f64<false, true, true>), grid, dim3(kLanes), 0, s, p);
-    else hipLaunchKernelGGL((k_step_f64<false, false, true>), grid, dim3(kLanes), 0, s, p);
+    if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false, true, kW4>), grid, blk, 0, s, p);
+    else hipLaunchKernelGGL((k_step_f64<false, false, true, kW4>), grid, blk, 0, s, p);
   } else {
-    if (p.open_bc && bc) hipLaunchKernelGGL((k_step_f64<true, true, false>), grid, dim3(kLanes), 0, s, p);
-    else if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false, false>), grid, dim3(kLanes), 0, s, p);
-    else if (bc) hipLaunchKernelGGL((k_step_f64<false, true, false>), grid, dim3(kLanes), 0, s, p);
-    else hipLaunchKernelGGL((k_step_f64<false, false, false>), grid, dim3(kLanes), 0, s, p);
+    if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, false, false, kW4>), grid, blk, 0, s, p);
+    else hipLaunchKernelGGL((k_step_f64<false, false, false, kW4>), grid, blk, 0, s, p);
   }
-  hipLaunchKernelGGL(k_close_f64, dim3(1), dim3(kLanes), 0, s, p, 1);
+}
+
+int launch_step_f64(const StepParams<double>& p, hipStream_t s) {
+  const dim3 grid(p.nblocks), blk(kLanes);
+  if (p.bc.n > 0) {
+    // boundary segments: the three-wave build only (four waves would spill
+    // the segment carry)
+    if (p.round32) {
+      if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, true, true, false>), grid, blk, 0, s, p);
+      else hipLaunchKernelGGL((k_step_f64<false, true, true, false>), grid, blk, 0, s, p);
+    } else {
+      if (p.open_bc) hipLaunchKernelGGL((k_step_f64<true, true, false, false>), grid, blk, 0, s, p);
+      else hipLaunchKernelGGL((k_step_f64<false, true, false, false>), grid, blk, 0, s, p);
+    }
+  } else if (p.waves4) {
+    launch_step_f64_w<true>(p, s);
+  } else {
+    launch_step_f64_w<false>(p, s);
+  }
+  hipLaunchKernelGGL(k_close_f64, dim3(1), blk, 0, s, p, 1);
   return launch_check("step_f64");
 }
 

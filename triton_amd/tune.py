@@ -4,7 +4,8 @@ The batched FP32 step has two knobs that change speed but not results: the
 number of waves per SIMD the row partition is cut into (which sets the band
 height of each wave's tile: the smallest height whose band-strip tiles need
 no more waves than that), and how many steps are captured per hipGraph.
-The fused FP64 step has one: the rows per wave of its strips.
+The fused FP64 step has two: the rows per wave of its strips, and the build
+of its kernel (for three waves per SIMD, or four with a few values spilled).
 autotune_batch() and autotune_sim() time each candidate on the work itself
 (real steps, nothing is thrown away), keep the fastest and remember it per
 (device, member count, total cells) bucket in a small JSON cache
@@ -27,8 +28,12 @@ from pathlib import Path
 # when a step is short (small batches) and little otherwise (within 1 % on
 # the ensemble).
 DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0), (20, 0), (25, 0), (5, 10), (15, 10), (20, 10), (25, 10))
-# rows per wave of the fused FP64 step
+# rows per wave of the fused FP64 step, and its builds (waves per SIMD): on
+# the spun-up 4096^2 watershed the four-wave build at the default strips
+# measured 774 against 836 us per step, on Thacker 1024^2 65.4 against
+# 57.2 us, so each grid takes the faster one
 DEFAULT_STRIPS = (4, 8, 16, 24, 32, 48, 64, 96, 128)
+DEFAULT_WAVES = (3, 4)
 
 
 def cache_path() -> Path:
@@ -123,27 +128,33 @@ def time_sim_steps(sim, steps: int) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def autotune_sim(sim, candidates=None, steps: int = 8, cache: bool = True) -> int:
-    """Pick the strip height (rows per wave) of a fused FP64 simulation
-    among the candidates and the simulation's size-based default; returns
-    it.  The simulation advances by 2 * steps per candidate at most."""
+def autotune_sim(sim, candidates=None, steps: int = 8, cache: bool = True, waves=None) -> int:
+    """Pick the strip height (rows per wave) and kernel build (waves per SIMD)
+    of a fused FP64 simulation among the candidates and the simulation's
+    size-based default strip; sets both and returns the strip height.  The
+    simulation advances by 2 * steps per (strip, build) pair at most."""
     c = sim.case
     key = f"{_device()}|f64|{c.rows}x{c.cols}"
     table = _load() if cache else {}
-    if key in table:
+    if key in table and "waves" in table[key]:
         rows = int(table[key]["strip_rows"])
         sim.set_strip_rows(rows)
+        sim.set_waves(int(table[key]["waves"]))
         return rows
     results = []
     cands = list(candidates or DEFAULT_STRIPS)
     if sim.strip_rows not in cands:
         cands.append(sim.strip_rows)  # the size-based default competes too
-    for rows in cands:
-        sim.set_strip_rows(rows)
-        results.append((time_sim_steps(sim, steps), int(rows)))
-    best = min(results)[1]
+    for w in (waves or DEFAULT_WAVES):
+        sim.set_waves(w)
+        for rows in cands:
+            sim.set_strip_rows(rows)
+            results.append((time_sim_steps(sim, steps), int(rows), int(w)))
+    _, best, best_w = min(results)
     sim.set_strip_rows(best)
+    sim.set_waves(best_w)
     if cache:
-        table[key] = {"strip_rows": best, "ms": {str(r): round(t, 5) for t, r in results}}
+        table[key] = {"strip_rows": best, "waves": best_w,
+                      "ms": {f"{r},{w}": round(t, 5) for t, r, w in results}}
         _store(table)
     return best
