@@ -787,8 +787,9 @@ int tr_batch_sync(tr_batch* b, int32_t* n_active) {
     set_error("tr_batch_sync: NULL batch");
     return TR_ERR_INVALID;
   }
-  TR_HIP(hipStreamSynchronize(b->stream));
-  RC(read_ctrls(b));
+  // the control copy is queued behind the steps, so its wait is the sync
+  if (b->cap == 0) TR_HIP(hipStreamSynchronize(b->stream));
+  else RC(read_ctrls(b));
   int act = 0;
   for (int i = 0; i < int(b->m.size()); i++) {
     Member& mm = b->m[size_t(i)];

@@ -79,6 +79,7 @@ struct tr_serve {
   int32_t next_id = 0;
   tr_serve_stats stats{};
   std::vector<double> latencies;
+  bool rearm = false;  // a member stopped at a print: resume it before the next chunk
 };
 
 namespace {
@@ -147,6 +148,7 @@ int capture_prints(tr_serve* s) {
       }
     }
     r.last_print = p.print_id;
+    s->rearm = true;
     if (r.keep_prints) r.prints.push_back(std::move(p));
   }
   return TR_OK;
@@ -336,7 +338,13 @@ int tr_serve_step(tr_serve* s, int32_t* ran) {
   if (s->running.empty()) return TR_OK;
   const double t0 = now_s();
   int32_t active = 0;
-  if ((rc = tr_batch_arm(s->batch, 0.0, s->opt.chunk_steps))) return rc;
+  // Members run to their horizon with no step limit (as tr_batch_add leaves
+  // them), so a round re-arms the batch only to resume members that stopped
+  // at a print the last round captured.
+  if (s->rearm) {
+    if ((rc = tr_batch_arm(s->batch, 0.0, -1))) return rc;
+    s->rearm = false;
+  }
   if ((rc = tr_batch_enqueue(s->batch, s->opt.chunk_steps))) return rc;
   if ((rc = tr_batch_sync(s->batch, &active))) return rc;
   if ((rc = capture_prints(s))) return rc;
