@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving leg (Poisson arrivals; 0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.8,
-                    help="offered load of the serving leg against the measured batched GCUPS")
+                    help="offered load of the serving leg against its measured capacity (the requests batched)")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0, help="HBM budget of the serving leg")
     ap.add_argument("--launch", default=None,
                     help="ensemble: fixed launch shape 'waves_per_simd,graph_steps' instead of the autotuner "
@@ -331,17 +331,21 @@ def fp64_plain(members, spinup_s, warmup, steps, keep=None):
             "autotuned_waves_per_simd": {str(w): sum(1 for _, x in strips if x == w) for w in (3, 4)}}
 
 
-def serving(args, capacity_gcups, waves_per_simd=0):
+def serving(args, ensemble_gcups, waves_per_simd=0):
     """Serving leg: Poisson arrivals (seed 5) of watershed requests, sides
     round(2^U(8,12)), horizons U(1,6) h, request seeds 5000+k, rain as in the
-    watershed generator; continuous batching under an HBM budget.  The rate
-    is set for the requested offered load from an a-priori work estimate of
-    each request (interior cells x horizon / 1.2 s, the CFL dt of 1 m deep
-    water moving at 1 m/s on the 10 m grid); the line also reports the load
-    the stream turned out to carry (achieved_load) and its work against that
-    estimate.  The batch takes the launch shape (waves per SIMD) the
-    ensemble was tuned to."""
+    watershed generator; continuous batching under an HBM budget.
+
+    The arrival rate is set for the requested offered load against measured
+    capacity: every request is first stepped to its horizon in one batch
+    (a request's steps do not depend on the batch it runs in, so this gives
+    each request's exact work in cell-steps), and the capacity is that run's
+    rate on this mix of sizes with all of it resident.  Then
+    rate = load x capacity / mean work.  The batch takes the launch shape
+    (waves per SIMD) the ensemble was tuned to."""
     import numpy as np
+    import torch
+    import triton_amd as ta
     from triton_amd import scenarios
     from triton_amd.serve import serve_poisson
     rng = np.random.default_rng(5)
@@ -352,8 +356,18 @@ def serving(args, capacity_gcups, waves_per_simd=0):
     cases = [scenarios.watershed(n=sd, seed=5000 + k, hours=hr, inflow_peaks=())
              for k, (sd, hr) in enumerate(zip(sides, hours))]
     log(f"generated {n} requests in {time.perf_counter() - t0:.1f} s")
-    est = np.array([sd * sd * hr * 3600.0 / 1.2 for sd, hr in zip(sides, hours)])
-    rate = args.serve_load * capacity_gcups * 1e9 / est.mean()
+    with ta.Batch(waves_per_simd=waves_per_simd) as cal:
+        ids = [cal.add(c) for c in cases]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cal.run()
+        torch.cuda.synchronize()
+        t_cal = time.perf_counter() - t0
+        work = np.array([float(interior(c)) * cal.poll(i)["steps"] for c, i in zip(cases, ids)])
+    capacity = work.sum() / t_cal / 1e9
+    log(f"capacity on this stream: {capacity:.2f} GCUPS ({work.sum():.4g} cell-steps in {t_cal:.1f} s, "
+        f"all requests resident)")
+    rate = args.serve_load * capacity * 1e9 / work.mean()
     budget = int(args.serve_budget_gb * 2 ** 30)
     t0 = time.perf_counter()
     srv, offsets = serve_poisson(cases, rate, seed=5, hbm_budget_bytes=budget, chunk_steps=32,
@@ -361,15 +375,16 @@ def serving(args, capacity_gcups, waves_per_simd=0):
     wall = time.perf_counter() - t0
     st = srv.stats
     out = {"requests": n, "arrival_rate_per_s": round(rate, 4), "offered_load": args.serve_load,
+           "capacity_gcups": round(capacity, 3), "ensemble_gcups": round(ensemble_gcups, 3),
+           "work_cell_steps": float(work.sum()),
            "hbm_budget_gb": args.serve_budget_gb, "p50_latency_s": round(st.percentile(50), 3),
            "p99_latency_s": round(st.percentile(99), 3), "max_latency_s": round(max(st.latencies), 3),
            "gcups_wall": round(st.cell_steps / wall / 1e9, 3),
            "gcups_busy": round(st.cell_steps / max(st.busy_s, 1e-9) / 1e9, 3),
-           # what the stream turned out to be: served cell-steps over the wall
-           # time against the batched capacity, and the served work against
-           # the a-priori estimate the rate was set from
-           "achieved_load": round(st.cell_steps / wall / 1e9 / capacity_gcups, 3),
-           "work_over_estimate": round(st.cell_steps / float(est.sum()), 3),
+           # served cell-steps over the wall time against the capacity, and
+           # the served work against the calibration run's (1.0: the same steps)
+           "achieved_load": round(st.cell_steps / wall / 1e9 / capacity, 3),
+           "work_served_over_calibrated": round(st.cell_steps / float(work.sum()), 4),
            "wall_s": round(wall, 2), "max_concurrent": st.max_members,
            "peak_gb": round(st.peak_bytes / 2 ** 30, 3), "rounds": st.rounds, "chunk_steps": 32,
            "waves_per_simd": waves_per_simd,
