@@ -632,13 +632,14 @@ int fused_grid_x_f32(int cols) { return (cols - 2 + kOutCols - 1) / kOutCols; }
 
 // Rows per wave of the FP64 fused step: long strips amortise the two-row
 // halo and the per-wave set-up (measured best: 96 rows on a 4096^2 grid,
-// about 3 waves per SIMD), short ones give a small grid enough waves (4 rows
-// on 400^2: 37 us per step against 71 us with 16).
+// about 3 waves per SIMD), short ones give a small grid enough waves (on
+// 400^2: 26.1 us per step with 3 rows, 29.4 with 4, 29.1 with 2, 32.7 with 1
+// and 35.9 with 6).
 int default_strip_rows(int rows, int cols) {
   const int nbx = fused_grid_x_f32(cols);
   const int64_t target_waves = 3072;
   int64_t strip = (int64_t(rows - 2) * nbx + target_waves - 1) / target_waves;
-  return strip < 4 ? 4 : (strip > 128 ? 128 : int(strip));
+  return strip < 3 ? 3 : (strip > 128 ? 128 : int(strip));
 }
 
 int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s) {

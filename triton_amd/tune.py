@@ -32,7 +32,7 @@ DEFAULT_CANDIDATES = ((5, 0), (10, 0), (15, 0), (20, 0), (25, 0), (5, 10), (15, 
 # the spun-up 4096^2 watershed the four-wave build at the default strips
 # measured 774 against 836 us per step, on Thacker 1024^2 65.4 against
 # 57.2 us, so each grid takes the faster one
-DEFAULT_STRIPS = (4, 8, 16, 24, 32, 48, 64, 96, 128)
+DEFAULT_STRIPS = (3, 4, 8, 16, 24, 32, 48, 64, 96, 128)
 DEFAULT_WAVES = (3, 4)
 
 
