@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batch.hpp"
@@ -199,6 +200,24 @@ int m_alloc(Member& mm, T*& p, size_t n) {
   mm.allocs.push_back(p);
   mm.bytes += int64_t(n * sizeof(T));
   return TR_OK;
+}
+
+// f(lo, hi) over [0, n) split across host threads, at least `grain` items
+// each: admitting a large member converts a few 16 M-cell arrays, which
+// single-threaded is most of its 175 ms on a 4087^2 grid.
+template <class F>
+void host_parallel(size_t n, size_t grain, F f) {
+  grain = std::max<size_t>(grain, 1);
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), (n + grain - 1) / grain);
+  if (nt <= 1) {
+    f(size_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (size_t t = 0; t < nt; t++) th.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+  for (auto& x : th) x.join();
 }
 
 template <class T>
@@ -506,23 +525,32 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   P.hextra_f = float(v.hextra);
   P.cn_dx_f = float(v.courant * v.dx);
   // FP32 state; the bed as per-face differences taken in FP64
-  std::vector<float> hf(n), qxf(n), qyf(n), dzx(n, 0.f), dzy(n, 0.f), n2f;
-  for (size_t i = 0; i < n; i++) {
-    hf[i] = float(v.h[i]);
-    qxf[i] = float(v.qx[i]);
-    qyf[i] = float(v.qy[i]);
-  }
-  for (int r = 0; r < rows; r++)
-    for (int c = 0; c < cols; c++) {
-      const size_t i = size_t(r) * cols + c;
-      if (c < cols - 1) dzx[i] = float(v.dem[i + 1] - v.dem[i]);
-      if (r >= 1) dzy[i] = float(v.dem[i - size_t(cols)] - v.dem[i]);
-    }
-  int rc = TR_OK;
-  for (int k = 0; k < 2 && !rc; k++) {
-    rc = m_upload(mm, P.h[k], hf.data(), n, s);
-    if (!rc) rc = m_upload(mm, P.qx[k], qxf.data(), n, s);
-    if (!rc) rc = m_upload(mm, P.qy[k], qyf.data(), n, s);
+  std::vector<float> hf(n), qxf(n), qyf(n), dzx(n), dzy(n), n2f(fric ? n : 0);
+  host_parallel(size_t(rows), (size_t(1) << 20) / size_t(cols) + 1, [&](size_t r0, size_t r1) {
+    for (size_t r = r0; r < r1; r++)
+      for (int c = 0; c < cols; c++) {
+        const size_t i = r * size_t(cols) + size_t(c);
+        hf[i] = float(v.h[i]);
+        qxf[i] = float(v.qx[i]);
+        qyf[i] = float(v.qy[i]);
+        dzx[i] = c < cols - 1 ? float(v.dem[i + 1] - v.dem[i]) : 0.f;
+        dzy[i] = r >= 1 ? float(v.dem[i - size_t(cols)] - v.dem[i]) : 0.f;
+        if (fric) n2f[i] = float(v.nman2[i]);
+      }
+  });
+  // the second ping-pong slot starts as a device copy of the first
+  int rc = m_upload(mm, P.h[0], hf.data(), n, s);
+  if (!rc) rc = m_upload(mm, P.qx[0], qxf.data(), n, s);
+  if (!rc) rc = m_upload(mm, P.qy[0], qyf.data(), n, s);
+  if (!rc) rc = m_alloc(mm, P.h[1], n);
+  if (!rc) rc = m_alloc(mm, P.qx[1], n);
+  if (!rc) rc = m_alloc(mm, P.qy[1], n);
+  if (!rc && n) {
+    const size_t nb = n * sizeof(float);
+    hipError_t e = hipMemcpyAsync(P.h[1], P.h[0], nb, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(P.qx[1], P.qx[0], nb, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(P.qy[1], P.qy[0], nb, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "tr_batch_add: slot copy");
   }
   float* pdx = nullptr;
   float* pdy = nullptr;
@@ -532,8 +560,6 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
   P.dzy = pdy;
   P.nman2 = nullptr;
   if (!rc && fric) {
-    n2f.resize(n);
-    for (size_t i = 0; i < n; i++) n2f[i] = float(v.nman2[i]);
     float* p = nullptr;
     rc = m_upload(mm, p, n2f.data(), n, s);
     P.nman2 = p;
@@ -553,8 +579,10 @@ int tr_batch_add_ex(tr_batch* b, const tr_case_view* vp, const tr_tuning* tune, 
     rc = m_upload(mm, rate, v.roff_rate, size_t(v.num_runoffs) * v.roff_nrows, s);
     if (!rc) rc = m_upload(mm, rt, v.roff_time, size_t(v.roff_nrows), s);
     std::vector<double> area(size_t(v.num_runoffs), 0.0);
-    for (int r = 1; r < rows - 1; r++)
-      for (int c = 1; c < cols - 1; c++) area[size_t(v.runoff_id ? v.runoff_id[size_t(r) * cols + c] : 0)] += 1.0;
+    if (!zones) area[0] = double(rows - 2) * double(cols - 2);  // every interior cell in zone 0
+    else
+      for (int r = 1; r < rows - 1; r++)
+        for (int c = 1; c < cols - 1; c++) area[size_t(v.runoff_id[size_t(r) * cols + c])] += 1.0;
     for (auto& a : area) a *= v.dx * v.dx;
     if (!rc) rc = m_upload(mm, za, area.data(), area.size(), s);
     P.roff_rate = rate;
