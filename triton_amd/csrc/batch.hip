@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -216,7 +217,13 @@ void host_parallel(size_t n, size_t grain, F f) {
   }
   std::vector<std::thread> th;
   th.reserve(nt);
-  for (size_t t = 0; t < nt; t++) th.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+  size_t t = 0;
+  try {
+    for (; t < nt; t++) th.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+  } catch (const std::system_error&) {
+    // no thread to spare: this thread takes the ranges not yet started
+    for (; t < nt; t++) f(n * t / nt, n * (t + 1) / nt);
+  }
   for (auto& x : th) x.join();
 }
 
