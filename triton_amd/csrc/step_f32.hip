@@ -109,11 +109,7 @@ __device__ __forceinline__ void face32(const Side& L, const Side& R, float dz, f
     const float w = ((R.qn - L.qn) - ub * dh) * ic;
     const float a0 = 0.5f * (dh - w), a2 = 0.5f * (dh + w);
     const float hp = L.h + a0;
-    // no entropy fix: fix0 needs aR0 > 0 and fix2 needs aL2 < 0, so two of
-    // their four tests rule both out (a face they leave in doubt takes the
-    // general path)
-    const bool common = !(aR0 > 0.f) && !(aL2 < 0.f) && l0 < 0.f && l2 > 0.f && hp > 0.f && L.h >= eps &&
-                        R.h >= eps;
+    const bool common = !fix0 && !fix2 && l0 < 0.f && l2 > 0.f && hp > 0.f && L.h >= eps && R.h >= eps;
     if (__all(common)) {
       const float t1 = ub * ((R.qt - L.qt) - vb * dh);
       const bool up = dz >= 0.f;
