@@ -294,9 +294,10 @@ int tr_sim_set_obs_halt(tr_sim *s, int32_t on);
 int tr_sim_set_strip_rows(tr_sim *s, int32_t strip_rows);
 int32_t tr_sim_strip_rows(const tr_sim *s);
 /* Fused FP64 forms: the build of the step kernel for the launches queued
- * next, 3 waves per SIMD (0 selects it; the default) or 4 (fewer registers,
- * a few values spilled); results do not depend on it (the autotuner's other
- * knob).  tr_sim_waves: the current value. */
+ * next, 3 waves per SIMD or 4 (fewer registers, a few values spilled); 0
+ * selects the default by grid size (4 where the three-wave strips would
+ * reach 40 rows, about 2700^2 and up; else 3).  Results do not depend on it
+ * (the autotuner's other knob).  tr_sim_waves: the current value. */
 int tr_sim_set_waves(tr_sim *s, int32_t waves_per_simd);
 int32_t tr_sim_waves(const tr_sim *s);
 /* The hipStream_t the simulation runs on. */
@@ -367,8 +368,9 @@ int tr_batch_add_ex(tr_batch *b, const tr_case_view *v, const tr_tuning *tune, i
  * be written out at print times as the reference's loop does
  * (src/triton_simulate.h:92-110). */
 int tr_batch_set_print_halt(tr_batch *b, int32_t id, int32_t on);
-/* Default rows per wave of the FP64 fused step for a padded rows x cols grid
- * (enough waves to fill the device; long strips otherwise). */
+/* Default rows per wave of the FP64 fused step for a padded rows x cols grid,
+ * for the default build (enough waves to fill the device; long strips
+ * otherwise). */
 int32_t tr_default_strip_rows(int32_t rows, int32_t cols);
 int tr_batch_remove(tr_batch *b, int32_t id);
 /* Limits for the steps queued next, for every member: t_stop (<= 0: each

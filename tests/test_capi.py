@@ -66,3 +66,14 @@ def test_noalloc_refusals_without_device():
     dc.h[1] = None
     assert f(C.byref(dc), 0.0, -1, C.byref(st), None) == ta.TR_ERR_INVALID
     assert b"state buffers" in ta.lib().tr_last_error()
+
+
+def test_default_strip_rows_by_grid_size():
+    """Rows per wave of the fused FP64 step when no strip is set (a host-side
+    rule, no device): about 3072 waves of at least 3 rows for the three-wave
+    build; grids whose strips would reach 40 rows take the four-wave build
+    with about 7168 waves of 24..48 rows.  Padded sizes (interior + 2)."""
+    f = ta.lib().tr_default_strip_rows
+    sizes = {402: 3, 1026: 6, 2050: 24, 2700: 24, 4098: 40}
+    assert {n: f(n, n) for n in sizes} == sizes
+    assert f(2, 2) == 0  # no interior: no strip

@@ -90,6 +90,7 @@ int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s);
 int launch_prime_f32(const InstF32* d_inst, CtrlPair* d_ctrls, int member, const InstF32& host_inst,
                      hipStream_t s);
 int fused_grid_x_f32(int cols);
-int default_strip_rows(int rows, int cols);
+int default_strip_rows(int rows, int cols);       // FP64 fused step, for its default build
+int default_waves_per_simd(int rows, int cols);  // that build: 3 or 4 waves per SIMD
 
 }  // namespace tr

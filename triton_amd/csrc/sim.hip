@@ -26,6 +26,7 @@ int copy_obs_f64(int size, const double* arr, double* obs, const int* index, hip
 int launch_prime_f64(const StepParams<double>& p, hipStream_t s);
 int launch_step_f64(const StepParams<double>& p, hipStream_t s);
 int fused_grid_x(int cols);
+int default_waves_per_simd(int rows, int cols);
 }  // namespace tr
 
 using namespace tr;
@@ -552,6 +553,7 @@ int tr_sim_create_ex(const tr_case_view* view, int32_t form, const tr_tuning* tu
   const tr_case_view& v = *view;
   s->strip_rows = (tune && tune->strip_rows > 0) ? tune->strip_rows
                                                  : tr_default_strip_rows(v.rows, v.cols);
+  s->waves4 = v.rows >= 3 && v.cols >= 3 && default_waves_per_simd(v.rows, v.cols) == 4 ? 1 : 0;
   const size_t n = size_t(v.rows) * size_t(v.cols);
   auto fail = [&](int rc) {
     delete s;
@@ -850,7 +852,8 @@ int tr_sim_set_waves(tr_sim* s, int32_t waves_per_simd) {
     return TR_ERR_STATE;
   }
   TR_HIP(hipStreamSynchronize(s->stream));
-  s->waves4 = waves_per_simd == 4 ? 1 : 0;
+  s->waves4 = waves_per_simd == 4 || (waves_per_simd == 0 && default_waves_per_simd(s->v.rows, s->v.cols) == 4)
+                  ? 1 : 0;
   return TR_OK;
 }
 
@@ -1043,6 +1046,7 @@ extern "C" int tr_sim_run_f64_noalloc(const tr_dev_case_f64* dc, double t_stop, 
   p.rows = c.rows;
   p.cols = c.cols;
   p.strip_rows = strip;
+  p.waves4 = default_waves_per_simd(c.rows, c.cols) == 4 ? 1 : 0;
   p.nbx = fused_grid_x(c.cols);
   p.nby = (c.rows - 2 + strip - 1) / strip;
   p.nblocks = unsigned(p.nbx) * unsigned(p.nby);

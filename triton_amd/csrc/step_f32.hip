@@ -630,16 +630,31 @@ __global__ __launch_bounds__(64) void k_close_f32(const InstF32* inst, CtrlPair*
 
 int fused_grid_x_f32(int cols) { return (cols - 2 + kOutCols - 1) / kOutCols; }
 
-// Rows per wave of the FP64 fused step: long strips amortise the two-row
-// halo and the per-wave set-up (measured best: 96 rows on a 4096^2 grid,
-// about 3 waves per SIMD), short ones give a small grid enough waves (on
-// 400^2: 26.1 us per step with 3 rows, 29.4 with 4, 29.1 with 2, 32.7 with 1
-// and 35.9 with 6).
-int default_strip_rows(int rows, int cols) {
+// Rows per wave of the FP64 fused step's three-wave build: long strips
+// amortise the two-row halo and the per-wave set-up (measured best: 96 rows
+// on a 4096^2 grid, about 3 waves per SIMD), short ones give a small grid
+// enough waves (on 400^2: 26.1 us per step with 3 rows, 29.4 with 4, 29.1
+// with 2, 32.7 with 1 and 35.9 with 6).
+static int strip_rows_w3(int rows, int cols) {
   const int nbx = fused_grid_x_f32(cols);
   const int64_t target_waves = 3072;
   int64_t strip = (int64_t(rows - 2) * nbx + target_waves - 1) / target_waves;
   return strip < 3 ? 3 : (strip > 128 ? 128 : int(strip));
+}
+
+// Grids whose three-wave strips would reach 40 rows run the four-wave build
+// (128 VGPRs, 12 B of spills) with about 7168 waves of 24..48 rows: on the
+// spun-up watershed 608-614 against 840 us per step at 4096^2 (40-row
+// strips) and 382-385 against 466 us at 3000^2 (24-40 rows); at 2048^2 (a
+// 24-row three-wave default) 189 against 186 us, so smaller grids keep the
+// three-wave build.
+int default_waves_per_simd(int rows, int cols) { return strip_rows_w3(rows, cols) >= 40 ? 4 : 3; }
+
+int default_strip_rows(int rows, int cols) {
+  if (default_waves_per_simd(rows, cols) == 3) return strip_rows_w3(rows, cols);
+  const int64_t target_waves = 7168;
+  const int64_t strip = (int64_t(rows - 2) * fused_grid_x_f32(cols) + target_waves - 1) / target_waves;
+  return strip < 24 ? 24 : (strip > 48 ? 48 : int(strip));
 }
 
 int launch_step_f32(const BatchArgs& a, int nwaves, hipStream_t s) {
